@@ -1,0 +1,150 @@
+/*
+ * wx_capi.h -- C-ABI of the MI355X (gfx950) implementation of Walicxe-2D's
+ * per-block hydrodynamic update.
+ *
+ * The reference (esquivas/walicxe2d, Fortran 90) has no FFI: its "plugin"
+ * mechanism is link-time shadowing of same-named subroutines from a problem
+ * directory (JET/Makefile:8, `VPATH = . : ../src`), all sharing module
+ * globals (src/globals.f90:8-33).  This header is the boundary a Fortran host
+ * binds through `iso_c_binding` (see fortran/wx_gpu.f90) so that drop-in
+ * `tstep` / `timestep` / `criteria` / `refine` / `coarse` routines call the
+ * GPU path instead of the CPU loops.
+ *
+ * Conventions
+ *   - every entry point returns int status: 0 = OK, otherwise a WX_E* code;
+ *     wx_last_error() returns the text of the last failure;
+ *   - plain pointers and sizes only; the library never keeps host pointers;
+ *   - block IDs are the reference's 1-based IDs (lp/leaf/icoord indices);
+ *   - one context = one AMR mesh resident on the current HIP device; calls
+ *     on one context are not re-entrant and run on the context's stream;
+ *   - wx_tstep and wx_step are asynchronous; wx_timestep, wx_criteria,
+ *     wx_admesh and wx_download synchronise.
+ */
+#ifndef WX_CAPI_H
+#define WX_CAPI_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WX_MAXLEV 16
+
+/* Status codes */
+#define WX_OK        0
+#define WX_EHLLC     1   /* HLLC/hybrid fell through (NaN star speed): src/hllcfluxes.f90:170-171 */
+#define WX_EBLOCKS   2   /* block capacity exceeded: src/admesh.f90:44-47 */
+#define WX_EARG      3   /* bad argument / inconsistent mesh */
+#define WX_EHIP      4   /* HIP runtime failure */
+#define WX_ENOMEM    5
+
+/* Layout selector for wx_upload / wx_download */
+#define WX_LAYOUT_BLOCK   0  /* [id][var][j][i], i fastest, id 0..cap (id 0 unused) */
+#define WX_LAYOUT_FORTRAN 1  /* u(nbmin:nbmaxtot, neq, -1:18, -1:18), block index fastest
+                                (src/initmain.f90:76-78) */
+
+/*
+ * Run-time parameters: the reference's cpp switches (JET/Makefile:118-209)
+ * and parameters.f90 constants (src/parameters.f90:12-89).
+ */
+typedef struct wx_params {
+  int32_t neq;          /* 4 + passive scalars (4..6) */
+  int32_t nlevs;        /* number of refinement levels */
+  int32_t nbrootx, nbrooty;
+  int32_t solver;       /* 0 HLLC (src/hllcfluxes.f90), 1 HLL (src/hllfluxes.f90),
+                           2 HLL_HLLC (src/hll_hllcfluxes.f90) */
+  int32_t limiter;      /* -1..6 (src/limiter.f90:30-85) */
+  int32_t cooling;      /* 0 ADIABATIC, 1 COOLINGH (src/coolingh.f90),
+                           2 COOLINGDMC (src/coolingdmc.f90) */
+  int32_t otherb;       /* 1: jet source via impose_user_bc in both ghost fills */
+  int32_t bc[4];        /* left,right,bottom,top: 0 OUTFLOW, 1 REFLECTIVE */
+  int32_t cylindrical;  /* axisymmetric source terms (src/step.f90:49-76) */
+  int32_t compat;       /* bit0: HLLC right-star typo (src/hllcfluxes.f90:157);
+                           bit1: OUTFYB writes block = list position (src/boundaryI.f90:87) */
+  int32_t nblocks;      /* block-ID capacity of the context */
+  int32_t pad0;
+  double cv, gamma, eta, cfl;
+  double xmax, ymax;
+  double Tempsc, Psc, tsc;
+  double tol, tol2;     /* refinement thresholds (src/criteria.f90:11) */
+  double dx[WX_MAXLEV], dy[WX_MAXLEV];
+  double cooltab[82];   /* DMC table cooltab(2,41), column-major (src/dmc_module.f90:9) */
+  double jet[12];       /* RJ, LJ, posx, posy, denJ, TempJ, vJ0, dvJ, tau, y, 2-y, unused
+                           (JET/jet.f90:15-35, code units) */
+} wx_params;
+
+typedef struct wx_ctx wx_ctx;
+
+/* Allocation of the block arrays and the DMC table: replaces
+ * src/initmain.f90:52-98 (nbmin/nbmaxtot, allocate u/up/primit, cooltab). */
+int wx_create(const wx_params *p, wx_ctx **out);
+int wx_destroy(wx_ctx *ctx);
+
+/* Host -> device copy of the mesh: u (layout per WX_LAYOUT_*), lp(nb,1:8),
+ * leaf(nb), icoord(nb,1:2) in the reference's Fortran layouts
+ * (src/globals.f90:13-18, column-major, ld = leading dimension = number of
+ * rows of lp/icoord/leaf arrays, rows indexed by block ID - 1).
+ * Replaces nothing in the reference (its arrays are already in memory);
+ * this is the transpose into the device's [id][var][j][i] tiles. */
+int wx_upload(wx_ctx *ctx, const double *u, int32_t layout, const int32_t *lp,
+              const int32_t *leaf, const int32_t *icoord, int32_t ld,
+              int32_t nbmax, int32_t nbleafs);
+/* Device -> host copy of u and of the primitives recomputed from it
+ * (calcprim, src/calcprim.f90:5-21).  prim may be NULL. */
+int wx_download(wx_ctx *ctx, double *u, double *prim, int32_t layout);
+/* Device -> host copy of the mesh tree kept by the context's own admesh. */
+int wx_get_tree(wx_ctx *ctx, int32_t *lp, int32_t *leaf, int32_t *icoord, int32_t ld,
+                int32_t *nbmax, int32_t *nbleafs);
+
+/* Ghost-fill list from the host's locatebounds (src/locatebounds.f90:29-540):
+ * innerbounds(ld,3) column-major, ninternal rows (codes -4..12, MPIP forms). */
+int wx_set_bounds(wx_ctx *ctx, const int32_t *innerbounds, int32_t ld, int32_t ninternal);
+/* Same list built by the library's own locatebounds from its tree. */
+int wx_locatebounds(wx_ctx *ctx, int32_t *ninternal);
+
+/* src/timestep.f90:7-61: CFL (+ DMC cooling-time) timestep over leaf
+ * interiors.  Synchronising. */
+int wx_timestep(wx_ctx *ctx, double *dt);
+
+/* src/tstep.f90:4-126: one full step (stage A, boundaryII, stage B +
+ * viscosity, boundaryI, cooling) at time = t^{n+1} (src/main.f90:88,95).
+ * Uses the ghost list last set by wx_set_bounds / wx_locatebounds.
+ * Asynchronous. */
+int wx_tstep(wx_ctx *ctx, double time, double dt);
+
+/* src/criteria.f90:4-53 over all leaves (markref, src/markref.f90:4-85):
+ * irefup/irefdown indexed by block ID - 1, length nblocks.  Synchronising. */
+int wx_criteria(wx_ctx *ctx, int32_t *irefup, int32_t *irefdown);
+
+/* Data parts of src/refine.f90:30-42 (injection over the padded block) and
+ * src/coarse.f90:182-366 (2x2 restriction incl. ghosts).  The caller keeps
+ * its own tree (updatelpup/updatelpdown) and calls wx_upload_tree after. */
+int wx_refine(wx_ctx *ctx, int32_t nb, int32_t s1, int32_t s2, int32_t s3, int32_t s4);
+int wx_coarse(wx_ctx *ctx, int32_t dad, int32_t s1, int32_t s2, int32_t s3, int32_t s4);
+int wx_upload_tree(wx_ctx *ctx, const int32_t *lp, const int32_t *leaf, const int32_t *icoord,
+                   int32_t ld, int32_t nbmax, int32_t nbleafs);
+
+/* Standalone driver (the library's own host tree):
+ *   wx_basegrid  -- generalised root layout (src/basegrid.f90:26-97) with the
+ *                   initial data taken from u (WX_LAYOUT_BLOCK, roots only);
+ *   wx_admesh    -- src/admesh.f90:4-111 (markref, critneighup, refine by
+ *                   level, critneighdown, coarsen by level);
+ *   wx_step      -- one iteration of the time loop src/main.f90:87-98:
+ *                   timestep, time += dt, tstep, admesh; time and dt stay
+ *                   on the device and are returned only when asked. */
+int wx_basegrid(wx_ctx *ctx, const double *uroots, int32_t nroots);
+int wx_admesh(wx_ctx *ctx, int32_t *changed);
+int wx_step(wx_ctx *ctx, int32_t nsteps, int32_t do_admesh, double *time, double *dt,
+            double *zone_updates);
+int wx_set_time(wx_ctx *ctx, double time);
+
+/* Bookkeeping */
+int wx_info(wx_ctx *ctx, int32_t *nbmax, int32_t *nbleafs, int32_t *ninternal);
+int wx_last_error(char *buf, int32_t len);
+/* HIP stream the context launches on (hipStream_t as void*). */
+void *wx_stream(wx_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,55 @@
+"""Builds the gfx950 shared library walicxe2d_amd/libwxhip.so in-tree.
+
+    python -m walicxe2d_amd.build        (or __graft_entry__.build())
+
+hipcc cross-compiles for gfx950 without a GPU present.  Floating-point
+contraction is off so the FP64 arithmetic keeps the reference's operation
+order; packed FP32 instructions are disabled for every HIP translation unit.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libwxhip.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+COMMON = [
+    "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+    "-Wall", "-Wno-unused-function",
+]
+DEVICE = [
+    "--offload-arch=gfx950", "-fno-gpu-rdc",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose: bool = True) -> str:
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    k_obj = os.path.join(objdir, "wx_kernels.o")
+    _run([HIPCC, *COMMON, *DEVICE, "-c", os.path.join(CSRC, "wx_kernels.hip"), "-o", k_obj])
+    objs.append(k_obj)
+    h_obj = os.path.join(objdir, "wx_host.o")
+    _run([HIPCC, *COMMON, *DEVICE, "-x", "hip", "-c", os.path.join(CSRC, "wx_host.cpp"), "-o", h_obj])
+    objs.append(h_obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    shutil.rmtree(objdir, ignore_errors=True)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

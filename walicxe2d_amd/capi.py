@@ -1,0 +1,230 @@
+"""ctypes binding of the C-ABI (include/wx_capi.h) and a host-side driver.
+
+`Simulation` mirrors the reference's driver-level interface (src/main.f90
+time loop: timestep -> time += dt -> tstep -> admesh) on one GPU-resident AMR
+mesh.  It refuses to run without the HIP library: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .params import NCELL, NP, Config, WxParams, initial_state
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libwxhip.so")
+
+WX_LAYOUT_BLOCK = 0
+WX_LAYOUT_FORTRAN = 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_lib = None
+
+EXPORTS = [
+    "wx_create", "wx_destroy", "wx_upload", "wx_download", "wx_get_tree", "wx_set_bounds",
+    "wx_locatebounds", "wx_timestep", "wx_tstep", "wx_criteria", "wx_refine", "wx_coarse",
+    "wx_upload_tree", "wx_basegrid", "wx_admesh", "wx_step", "wx_set_time", "wx_info",
+    "wx_last_error", "wx_stream",
+]
+
+
+class WxError(RuntimeError):
+    pass
+
+
+def load_library(path: str = LIBPATH) -> ctypes.CDLL:
+    """Loads libwxhip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise WxError(f"{path} is missing: build it with `python -m walicxe2d_amd.build` "
+                      "(the hot path has no CPU fallback)")
+    L = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    L.wx_create.argtypes = [ctypes.POINTER(WxParams), ctypes.POINTER(P)]
+    L.wx_destroy.argtypes = [P]
+    L.wx_upload.argtypes = [P, _dp, ctypes.c_int32, _ip, _ip, _ip, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+    L.wx_download.argtypes = [P, _dp, _dp, ctypes.c_int32]
+    L.wx_get_tree.argtypes = [P, _ip, _ip, _ip, ctypes.c_int32, _ip, _ip]
+    L.wx_set_bounds.argtypes = [P, _ip, ctypes.c_int32, ctypes.c_int32]
+    L.wx_locatebounds.argtypes = [P, _ip]
+    L.wx_timestep.argtypes = [P, _dp]
+    L.wx_tstep.argtypes = [P, ctypes.c_double, ctypes.c_double]
+    L.wx_criteria.argtypes = [P, _ip, _ip]
+    L.wx_refine.argtypes = [P] + [ctypes.c_int32] * 5
+    L.wx_coarse.argtypes = [P] + [ctypes.c_int32] * 5
+    L.wx_upload_tree.argtypes = [P, _ip, _ip, _ip, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+    L.wx_basegrid.argtypes = [P, _dp, ctypes.c_int32]
+    L.wx_admesh.argtypes = [P, _ip]
+    L.wx_step.argtypes = [P, ctypes.c_int32, ctypes.c_int32, _dp, _dp, _dp]
+    L.wx_set_time.argtypes = [P, ctypes.c_double]
+    L.wx_info.argtypes = [P, _ip, _ip, _ip]
+    L.wx_last_error.argtypes = [ctypes.c_char_p, ctypes.c_int32]
+    L.wx_stream.argtypes = [P]
+    L.wx_stream.restype = P
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(512)
+    load_library().wx_last_error(buf, 512)
+    return buf.value.decode(errors="replace")
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise WxError(f"{what} failed (status {rc}): {last_error()}")
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def tree_to_fortran(lp: np.ndarray, leaf: np.ndarray, icoord: np.ndarray, nbmax: int):
+    """Row-per-ID arrays (index 0 unused) -> Fortran column-major lp(ld,8),
+    leaf(ld), icoord(ld,2) with ld = nbmax."""
+    ld = max(nbmax, 1)
+    flp = np.ascontiguousarray(lp[1:ld + 1].T, dtype=np.int32)
+    fleaf = np.ascontiguousarray(leaf[1:ld + 1], dtype=np.int32)
+    fic = np.ascontiguousarray(icoord[1:ld + 1].T, dtype=np.int32)
+    return flp, fleaf, fic, ld
+
+
+class Simulation:
+    """One AMR mesh resident on the GPU behind the C-ABI."""
+
+    def __init__(self, cfg: Config, nblocks: int | None = None):
+        self.L = load_library()
+        self.cfg = cfg
+        if nblocks is not None:
+            cfg.nblocks = nblocks
+        self.params = cfg.params()
+        self.neq = cfg.neq
+        self.cap = cfg.nblocks
+        h = ctypes.c_void_p()
+        _check(self.L.wx_create(ctypes.byref(self.params), ctypes.byref(h)), "wx_create")
+        self.h = h
+        self.time = 0.0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.wx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- set-up -----------------------------------------------------
+    def basegrid(self):
+        """Root layout + initial conditions (src/basegrid.f90:26-97 generalised)."""
+        u, lp, icoord = initial_state(self.cfg)
+        nroots = lp.shape[0] - 1
+        _check(self.L.wx_basegrid(self.h, _d(np.ascontiguousarray(u)), nroots), "wx_basegrid")
+        self.time = 0.0
+        _check(self.L.wx_set_time(self.h, 0.0), "wx_set_time")
+
+    def upload(self, u: np.ndarray, lp: np.ndarray, leaf: np.ndarray, icoord: np.ndarray,
+               nbmax: int, nbleafs: int):
+        """u: [>= nbmax+1][neq][20][20] block layout; tree arrays row-per-ID."""
+        flp, fleaf, fic, ld = tree_to_fortran(lp, leaf, icoord, nbmax)
+        u = np.ascontiguousarray(u[:nbmax + 1], dtype=np.float64)
+        _check(self.L.wx_upload(self.h, _d(u), WX_LAYOUT_BLOCK, _i(flp), _i(fleaf), _i(fic), ld,
+                                nbmax, nbleafs), "wx_upload")
+
+    def upload_tree(self, lp, leaf, icoord, nbmax, nbleafs):
+        flp, fleaf, fic, ld = tree_to_fortran(lp, leaf, icoord, nbmax)
+        _check(self.L.wx_upload_tree(self.h, _i(flp), _i(fleaf), _i(fic), ld, nbmax, nbleafs),
+               "wx_upload_tree")
+
+    # ---- state --------------------------------------------------------
+    def info(self):
+        a, b, c = (ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32())
+        _check(self.L.wx_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)), "wx_info")
+        return a.value, b.value, c.value
+
+    def download(self, prim: bool = False):
+        nbmax = self.info()[0]
+        u = np.zeros((nbmax + 1, self.neq, NP, NP))
+        pr = np.zeros_like(u) if prim else None
+        _check(self.L.wx_download(self.h, _d(u), _d(pr) if prim else None, WX_LAYOUT_BLOCK), "wx_download")
+        return (u, pr) if prim else u
+
+    def download_fortran(self):
+        nbmax = self.info()[0]
+        u = np.zeros(nbmax * self.neq * NCELL)
+        _check(self.L.wx_download(self.h, _d(u), None, WX_LAYOUT_FORTRAN), "wx_download")
+        return u
+
+    def tree(self):
+        nbmax = self.info()[0]
+        ld = max(nbmax, 1)
+        flp = np.zeros((8, ld), dtype=np.int32)
+        fleaf = np.zeros(ld, dtype=np.int32)
+        fic = np.zeros((2, ld), dtype=np.int32)
+        a, b = ctypes.c_int32(), ctypes.c_int32()
+        _check(self.L.wx_get_tree(self.h, _i(flp), _i(fleaf), _i(fic), ld, ctypes.byref(a), ctypes.byref(b)),
+               "wx_get_tree")
+        lp = np.zeros((ld + 1, 8), dtype=np.int32)
+        leaf = np.zeros(ld + 1, dtype=np.int32)
+        icoord = np.zeros((ld + 1, 2), dtype=np.int32)
+        lp[1:] = flp.T
+        leaf[1:] = fleaf
+        icoord[1:] = fic.T
+        return lp, leaf, icoord, a.value, b.value
+
+    # ---- the reference's interfaces ----------------------------------
+    def set_bounds(self, bounds: np.ndarray):
+        """innerbounds rows (n,3) in list order."""
+        b = np.asarray(bounds, dtype=np.int32).reshape(-1, 3)
+        n = b.shape[0]
+        fb = np.ascontiguousarray(b.T) if n else np.zeros((3, 1), dtype=np.int32)
+        _check(self.L.wx_set_bounds(self.h, _i(fb), max(n, 1), n), "wx_set_bounds")
+
+    def locatebounds(self) -> int:
+        n = ctypes.c_int32()
+        _check(self.L.wx_locatebounds(self.h, ctypes.byref(n)), "wx_locatebounds")
+        return n.value
+
+    def timestep(self) -> float:
+        dt = np.zeros(1)
+        _check(self.L.wx_timestep(self.h, _d(dt)), "wx_timestep")
+        return float(dt[0])
+
+    def tstep(self, time: float, dt: float):
+        _check(self.L.wx_tstep(self.h, time, dt), "wx_tstep")
+
+    def criteria(self):
+        up = np.zeros(self.cap, dtype=np.int32)
+        dn = np.zeros(self.cap, dtype=np.int32)
+        _check(self.L.wx_criteria(self.h, _i(up), _i(dn)), "wx_criteria")
+        return up, dn
+
+    def refine(self, nb, s1, s2, s3, s4):
+        _check(self.L.wx_refine(self.h, nb, s1, s2, s3, s4), "wx_refine")
+
+    def coarse(self, dad, s1, s2, s3, s4):
+        _check(self.L.wx_coarse(self.h, dad, s1, s2, s3, s4), "wx_coarse")
+
+    def admesh(self) -> int:
+        ch = ctypes.c_int32()
+        _check(self.L.wx_admesh(self.h, ctypes.byref(ch)), "wx_admesh")
+        return ch.value
+
+    def step(self, nsteps: int = 1, admesh: bool = True):
+        """nsteps iterations of src/main.f90:87-98; returns (time, dt, zone_updates)."""
+        t, dt, z = np.zeros(1), np.zeros(1), np.zeros(1)
+        _check(self.L.wx_step(self.h, nsteps, 1 if admesh else 0, _d(t), _d(dt), _d(z)), "wx_step")
+        self.time = float(t[0])
+        return float(t[0]), float(dt[0]), float(z[0])

@@ -1,0 +1,83 @@
+// wx_device.h -- structures shared by the host runtime and the gfx950 kernels.
+#pragma once
+#include <stdint.h>
+#include "../../include/wx_capi.h"
+
+namespace wx {
+
+constexpr int NX = 16;
+constexpr int NY = 16;
+constexpr int NP = 20;      // padded tile extent (nghost = 2)
+constexpr int NC = NP * NP; // cells per padded tile
+
+// Ghost-segment op kinds.  A destination block's non-corner ghost ring is 4
+// sides x 2 halves ("segments", seg = side*2 + half; side 0 left, 1 right,
+// 2 bottom, 3 top; half 0 covers positions 1..8, half 1 positions 9..16).
+// Each segment is written by exactly one op per fill (the last writer of
+// the reference's ordered list, src/boundaryI.f90:34-396 and
+// src/boundaryII.f90:35-514); corners are never read by the scheme.
+enum GhostKind : int {
+  GK_NONE = 0,      // not written in this fill: keep the carried value
+  GK_COPY = 1,      // same level: neighbour interior strip
+  GK_RESTRICT = 2,  // finer neighbour: 2x2 means
+  GK_PROLONG = 3,   // coarser neighbour: 0th-order injection
+  GK_BOXOUT = 4,    // box boundary, outflow: mirror own interior
+  GK_BOXREF = 5,    // box boundary, reflective: mirror + normal velocity flip
+};
+// op.y bit fields
+constexpr int GF_SRC_IMPOSED = 1 << 4;   // source interior carried the jet at copy time
+constexpr int GF_DST_IMPOSED = 1 << 5;   // destination imposed after this write
+constexpr int GF_OFF_SHIFT = 8;          // PROLONG: 0 or 8 along the coarse side
+
+// Per-leaf record (one workgroup each).
+struct LeafRec {
+  int32_t id;      // block ID (tile slot)
+  int32_t level;
+  int32_t icx, icy;
+  int32_t flags;   // bit0: block imposed by the jet source in this step's fills
+  int32_t pad[3];
+};
+
+// Per-ID record for source lookups (level and lower-left cell index).
+struct BlockRec {
+  int32_t level, icx, icy, pad;
+};
+
+// Device scalars of one context.
+struct DevScalars {
+  double time;
+  double dt;
+  double dtp_min;
+  double dtcool_min;
+  int32_t err;
+  int32_t pad[7];
+};
+
+// Mesh data ops for admesh (src/refine.f90, src/coarse.f90).
+struct MeshOp {
+  int32_t kind;   // 0 refine (a = parent), 1 coarse (a = dad)
+  int32_t a, s1, s2, s3, s4;
+  int32_t pad[2];
+};
+
+// launch wrappers (wx_kernels.hip)
+int launch_stageA(const wx_params *dP, int neq, const LeafRec *leaves, int nleaf,
+                  const double *U, double *Q, DevScalars *S, void *stream);
+int launch_stageB(const wx_params *dP, int neq, const LeafRec *leaves, int nleaf,
+                  const int2 *ops, const BlockRec *binfo, double *U, const double *Q,
+                  DevScalars *S, void *stream);
+int launch_fillI_cool(const wx_params *dP, int neq, const LeafRec *leaves, int nleaf,
+                      const int2 *ops, const BlockRec *binfo, const double *Uin, double *Uout,
+                      double *stat, int8_t *flag, DevScalars *S, void *stream);
+int launch_blockstat(const wx_params *dP, int neq, const LeafRec *leaves, int nleaf,
+                     const double *U, double *stat, int8_t *flag, void *stream);
+int launch_dtreduce(const wx_params *dP, const double *stat, int nleaf, DevScalars *S,
+                    int advance_time, void *stream);
+int launch_meshops(const wx_params *dP, int neq, const MeshOp *ops, int nops, double *U,
+                   void *stream);
+int launch_transpose(int neq, int cap, const double *src, double *dst, int to_fortran,
+                     int ld, void *stream);
+int launch_calcprim(const wx_params *dP, int neq, int cap, const double *U, double *prim,
+                    void *stream);
+
+}  // namespace wx

@@ -1,0 +1,669 @@
+// wx_kernels.hip -- gfx950 kernels of the Walicxe-2D hot path.
+//
+// Data layout in HBM: one padded 20x20 tile per variable per block,
+// [id][var][j][i] with i fastest, id = the reference's block ID.  Every
+// kernel below assigns one 256-thread workgroup (4 wave64) to one leaf
+// block and stages the tile in LDS.
+//
+// One step (src/tstep.f90:4-126) is three launches over the leaf list:
+//   k_stageA      Uⁿ -> U^{n+1/2} interior (first-order fluxes, dt/2)
+//   k_stageB      pulls the 2-layer ghost ring of U^{n+1/2} from the
+//                 neighbours (boundaryII semantics), limited fluxes, update,
+//                 viscosity; writes U^{n+1} interior and the U^{n+1/2} ring
+//                 in place over Uⁿ (tstep.f90:85 copies the whole padded block)
+//   k_fillI_cool  pulls ghost layer 1 (boundaryI), imposes the jet, applies
+//                 cooling on cells 0..17, writes the final tile to the other
+//                 buffer, and emits the next CFL partials and the refinement
+//                 flag of the block (fused timestep/criteria epilogue).
+#include <hip/hip_runtime.h>
+#include "wx_device.h"
+#include "wx_physics.h"
+
+namespace wx {
+
+__device__ __forceinline__ int IDX(int i, int j) { return (j + 1) * NP + (i + 1); }
+
+// ---------------------------------------------------------------------
+// Ghost gather: value of one non-corner ghost cell of block `dst` at
+// (side, layer, pos) from the op that last wrote its segment.
+//   side 0 left (i = 1-layer), 1 right (i = 16+layer), 2 bottom (j = 1-layer),
+//   3 top (j = 16+layer); layer 1 is adjacent to the interior, 2 outer;
+//   pos 1..16 runs along the side.
+// Formulas: src/boundaryII.f90:143-500 and src/boundaryI.f90:109-382.
+template <int NEQ>
+__device__ __forceinline__ void src_cell(const wx_params *__restrict__ p, const double *__restrict__ A,
+                                         const BlockRec *__restrict__ binfo, int b, int i, int j,
+                                         bool imposed, double vjet, double *v) {
+  const double *t = A + (size_t)b * NEQ * NC + IDX(i, j);
+#pragma unroll
+  for (int k = 0; k < NEQ; ++k) v[k] = t[k * NC];
+  if (imposed) {
+    BlockRec r = binfo[b];
+    if (in_jet(p, r.level, r.icx, r.icy, i, j)) jet_state<NEQ>(p, vjet, v);
+  }
+}
+
+template <int NEQ>
+__device__ void ghost_gather(const wx_params *__restrict__ p, const double *__restrict__ A,
+                             const BlockRec *__restrict__ binfo, int2 op, int dst, int dlev,
+                             int dicx, int dicy, int side, int layer, int pos, double vjet,
+                             double *g) {
+  const int kind = op.y & 15;
+  const int src = op.x;
+  const bool simp = (op.y & GF_SRC_IMPOSED) != 0;
+  const bool dimp = (op.y & GF_DST_IMPOSED) != 0;
+  const int off = (op.y >> GF_OFF_SHIFT) & 15;
+  const bool xside = side < 2;
+  // destination cell
+  int di, dj;
+  if (side == 0) { di = 1 - layer; dj = pos; }
+  else if (side == 1) { di = NX + layer; dj = pos; }
+  else if (side == 2) { dj = 1 - layer; di = pos; }
+  else { dj = NY + layer; di = pos; }
+  if (kind == GK_COPY) {
+    int si = di, sj = dj;
+    if (side == 0) si = di + NX;
+    else if (side == 1) si = di - NX;
+    else if (side == 2) sj = dj + NY;
+    else sj = dj - NY;
+    src_cell<NEQ>(p, A, binfo, src, si, sj, simp, vjet, g);
+  } else if (kind == GK_RESTRICT) {
+    const int half = (pos - 1) >> 3;
+    int i1, j1;
+    if (xside) {
+      int jj = pos - 8 * half;
+      i1 = (side == 0) ? (2 * di - 1 + NX) : (2 * (di - NX) - 1);
+      j1 = 2 * jj - 1;
+    } else {
+      int ii = pos - 8 * half;
+      i1 = 2 * ii - 1;
+      j1 = (side == 2) ? (2 * dj - 1 + NY) : (2 * (dj - NY) - 1);
+    }
+    double a[NEQ], b2[NEQ], c[NEQ], d[NEQ];
+    src_cell<NEQ>(p, A, binfo, src, i1, j1, simp, vjet, a);
+    src_cell<NEQ>(p, A, binfo, src, i1 + 1, j1, simp, vjet, b2);
+    src_cell<NEQ>(p, A, binfo, src, i1, j1 + 1, simp, vjet, c);
+    src_cell<NEQ>(p, A, binfo, src, i1 + 1, j1 + 1, simp, vjet, d);
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) g[k] = 0.25 * (((a[k] + b2[k]) + c[k]) + d[k]);
+  } else if (kind == GK_PROLONG) {
+    int si, sj;
+    if (xside) {
+      si = (side == 0) ? NX : 1;
+      sj = (dj + 1) / 2 + off;
+    } else {
+      sj = (side == 2) ? NY : 1;
+      si = (di + 1) / 2 + off;
+    }
+    src_cell<NEQ>(p, A, binfo, src, si, sj, simp, vjet, g);
+  } else {  // GK_BOXOUT / GK_BOXREF : mirror own interior
+    int si = di, sj = dj;
+    if (side == 0) si = 1 - di;
+    else if (side == 1) si = 2 * NX + 1 - di;
+    else if (side == 2) sj = 1 - dj;
+    else sj = 2 * NY + 1 - dj;
+    src_cell<NEQ>(p, A, binfo, dst, si, sj, simp, vjet, g);
+    if (kind == GK_BOXREF) {
+      if (xside) g[1] = -g[1];
+      else g[2] = -g[2];
+    }
+  }
+  if (dimp && in_jet(p, dlev, dicx, dicy, di, dj)) jet_state<NEQ>(p, vjet, g);
+}
+
+__device__ __forceinline__ void ghost_coords(int side, int layer, int pos, int &i, int &j) {
+  if (side == 0) { i = 1 - layer; j = pos; }
+  else if (side == 1) { i = NX + layer; j = pos; }
+  else if (side == 2) { j = 1 - layer; i = pos; }
+  else { j = NY + layer; i = pos; }
+}
+
+// Flux sweep shared by both stages.  prim: LDS tile [NEQ][NC].  F: x-faces
+// fi=0..16 on rows 1..16 at [k][(j-1)*17+fi]; G: y-faces fj=0..16 on columns
+// 1..16 at [k][fj*16+(i-1)].  choice 2 reconstructs with the limiter.
+template <int NEQ, int CHOICE>
+__device__ __forceinline__ int flux_sweep(const wx_params *__restrict__ p, const double *prim,
+                                          double *F, double *G) {
+  int err = 0;
+  const int lim = p->limiter;
+  for (int f = threadIdx.x; f < 544; f += blockDim.x) {
+    double ql[NEQ], qr[NEQ], ff[NEQ];
+    if (f < 272) {
+      const int fi = f % 17, j = f / 17 + 1;
+#pragma unroll
+      for (int k = 0; k < NEQ; ++k) {
+        ql[k] = prim[k * NC + IDX(fi, j)];
+        qr[k] = prim[k * NC + IDX(fi + 1, j)];
+      }
+      if (CHOICE == 2) {
+        double qll[NEQ], qrr[NEQ];
+#pragma unroll
+        for (int k = 0; k < NEQ; ++k) {
+          qll[k] = prim[k * NC + IDX(fi - 1, j)];
+          qrr[k] = prim[k * NC + IDX(fi + 2, j)];
+        }
+        limiter<NEQ>(lim, qll, ql, qr, qrr);
+      }
+      err |= riemann<NEQ>(p, ql, qr, ff);
+#pragma unroll
+      for (int k = 0; k < NEQ; ++k) F[k * 272 + (j - 1) * 17 + fi] = ff[k];
+    } else {
+      const int g = f - 272;
+      const int i = g % 16 + 1, fj = g / 16;
+#pragma unroll
+      for (int k = 0; k < NEQ; ++k) {
+        ql[k] = prim[k * NC + IDX(i, fj)];
+        qr[k] = prim[k * NC + IDX(i, fj + 1)];
+      }
+      double t;
+      t = ql[1]; ql[1] = ql[2]; ql[2] = t;   // swapy, src/swapy.f90:4-13
+      t = qr[1]; qr[1] = qr[2]; qr[2] = t;
+      if (CHOICE == 2) {
+        double qll[NEQ], qrr[NEQ];
+#pragma unroll
+        for (int k = 0; k < NEQ; ++k) {
+          qll[k] = prim[k * NC + IDX(i, fj - 1)];
+          qrr[k] = prim[k * NC + IDX(i, fj + 2)];
+        }
+        t = qll[1]; qll[1] = qll[2]; qll[2] = t;
+        t = qrr[1]; qrr[1] = qrr[2]; qrr[2] = t;
+        limiter<NEQ>(lim, qll, ql, qr, qrr);
+      }
+      err |= riemann<NEQ>(p, ql, qr, ff);
+      t = ff[1]; ff[1] = ff[2]; ff[2] = t;
+#pragma unroll
+      for (int k = 0; k < NEQ; ++k) G[k * 272 + fj * 16 + (i - 1)] = ff[k];
+    }
+  }
+  return err;
+}
+
+// Cylindrical source terms, src/step.f90:49-76 (prims of the stage).
+template <int NEQ>
+__device__ __forceinline__ void cyl_source(const wx_params *__restrict__ p, const double *prim, int i,
+                                           int j, int icy, int lev, double *S) {
+  double radius = ((double)(j + icy) - 0.5) * p->dy[lev - 1];
+  double rho = prim[0 * NC + IDX(i, j)];
+  double v1 = prim[1 * NC + IDX(i, j)], v2 = prim[2 * NC + IDX(i, j)], pr = prim[3 * NC + IDX(i, j)];
+  double H = 0.5 * rho * (v1 * v1 + v2 * v2) + (p->cv + 1.) * pr;
+  S[0] = -(rho * v2 / radius);
+  S[1] = -(rho * v1 * v2 / radius);
+  S[2] = -(rho * v2 * v2 / radius);
+  S[3] = -(H * v2 / radius);
+#pragma unroll
+  for (int k = 4; k < NEQ; ++k) S[k] = -(prim[k * NC + IDX(i, j)] * v2 / radius);
+}
+
+// ---------------------------------------------------------------------
+// Stage A: src/tstep.f90:22-46 (hllcfluxes choice 1 + step with dt/2).
+template <int NEQ>
+__global__ __launch_bounds__(256) void k_stageA(const wx_params *__restrict__ p,
+                                                const LeafRec *__restrict__ leaves,
+                                                const double *__restrict__ U, double *__restrict__ Q,
+                                                DevScalars *__restrict__ S) {
+  __shared__ double prim[NEQ * NC];
+  __shared__ double F[NEQ * 272];
+  __shared__ double G[NEQ * 272];
+  const LeafRec L = leaves[blockIdx.x];
+  const double *ub = U + (size_t)L.id * NEQ * NC;
+  // rows j = 0..17 are one contiguous range [NP, 19*NP) of every variable
+  for (int c = threadIdx.x; c < 18 * NP; c += blockDim.x) {
+    double uu[NEQ], q[NEQ], T;
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) uu[k] = ub[k * NC + NP + c];
+    uprim<NEQ>(p, q, uu, T);
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) prim[k * NC + NP + c] = q[k];
+  }
+  __syncthreads();
+  int err = flux_sweep<NEQ, 1>(p, prim, F, G);
+  __syncthreads();
+  const double dtm = S->dt / 2.;
+  const double dtdx = dtm / p->dx[L.level - 1];
+  const double dtdy = dtm / p->dy[L.level - 1];
+  const int i = threadIdx.x % 16 + 1, j = threadIdx.x / 16 + 1;
+  double Sc[NEQ];
+  if (p->cylindrical) cyl_source<NEQ>(p, prim, i, j, L.icy, L.level, Sc);
+  double *qb = Q + (size_t)L.id * NEQ * NC;
+#pragma unroll
+  for (int k = 0; k < NEQ; ++k) {
+    double v = ub[k * NC + IDX(i, j)] - dtdx * (F[k * 272 + (j - 1) * 17 + i] - F[k * 272 + (j - 1) * 17 + i - 1])
+               - dtdy * (G[k * 272 + j * 16 + (i - 1)] - G[k * 272 + (j - 1) * 16 + (i - 1)]);
+    if (p->cylindrical) v = v + dtm * Sc[k];
+    qb[k * NC + IDX(i, j)] = v;
+  }
+  if (err) atomicOr(&S->err, WX_EHLLC);
+}
+
+// ---------------------------------------------------------------------
+// Stage B: src/tstep.f90:53-88 with boundaryII pulled from the neighbours.
+template <int NEQ>
+__global__ __launch_bounds__(256) void k_stageB(const wx_params *__restrict__ p,
+                                                const LeafRec *__restrict__ leaves,
+                                                const int2 *__restrict__ ops,
+                                                const BlockRec *__restrict__ binfo,
+                                                double *__restrict__ U, const double *__restrict__ Q,
+                                                DevScalars *__restrict__ S) {
+  __shared__ double prim[NEQ * NC];
+  __shared__ double F[NEQ * 272];
+  __shared__ double G[NEQ * 272];
+  const int l = blockIdx.x;
+  const LeafRec L = leaves[l];
+  const int b = L.id;
+  const double time = S->time;
+  const bool imp = (L.flags & 1) != 0;
+  const double vjet = p->otherb ? jet_velocity(p, time) : 0.0;
+  const double *qb = Q + (size_t)b * NEQ * NC;
+  double *ub = U + (size_t)b * NEQ * NC;
+  const int tid = threadIdx.x;
+  // interior of U^{n+1/2} (imposed when the block is in the source list)
+  {
+    const int i = tid % 16 + 1, j = tid / 16 + 1;
+    double uu[NEQ], q[NEQ], T;
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) uu[k] = qb[k * NC + IDX(i, j)];
+    if (imp && in_jet(p, L.level, L.icx, L.icy, i, j)) jet_state<NEQ>(p, vjet, uu);
+    uprim<NEQ>(p, q, uu, T);
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) prim[k * NC + IDX(i, j)] = q[k];
+  }
+  // two-layer ghost ring: 4 sides x 2 layers x 16 positions = 128 cells
+  double gh[NEQ];
+  int gi = 0, gj = 0;
+  const bool ghost = tid < 128;
+  if (ghost) {
+    const int side = tid >> 5, layer = ((tid >> 4) & 1) + 1, pos = (tid & 15) + 1;
+    const int2 op = ops[l * 8 + side * 2 + ((pos - 1) >> 3)];
+    ghost_gather<NEQ>(p, Q, binfo, op, b, L.level, L.icx, L.icy, side, layer, pos, vjet, gh);
+    ghost_coords(side, layer, pos, gi, gj);
+    double q[NEQ], T;
+    uprim<NEQ>(p, q, gh, T);
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) prim[k * NC + IDX(gi, gj)] = q[k];
+  }
+  __syncthreads();
+  int err = flux_sweep<NEQ, 2>(p, prim, F, G);
+  __syncthreads();
+  const double dt = S->dt;
+  const double dtdx = dt / p->dx[L.level - 1];
+  const double dtdy = dt / p->dy[L.level - 1];
+  const int i = tid % 16 + 1, j = tid / 16 + 1;
+  double Sc[NEQ];
+  if (p->cylindrical) cyl_source<NEQ>(p, prim, i, j, L.icy, L.level, Sc);
+  const bool visc = L.level >= p->nlevs - 1;   // src/tstep.f90:82
+  const double eta = p->eta;
+  double un[NEQ];
+#pragma unroll
+  for (int k = 0; k < NEQ; ++k) {
+    const double *uk = ub + k * NC;
+    double v = uk[IDX(i, j)] - dtdx * (F[k * 272 + (j - 1) * 17 + i] - F[k * 272 + (j - 1) * 17 + i - 1])
+               - dtdy * (G[k * 272 + j * 16 + (i - 1)] - G[k * 272 + (j - 1) * 16 + (i - 1)]);
+    if (p->cylindrical) v = v + dt * Sc[k];
+    if (visc)   // src/viscosity.f90:6-25 (no dt factor)
+      v = v + eta * (uk[IDX(i + 1, j)] + uk[IDX(i - 1, j)] + uk[IDX(i, j + 1)] + uk[IDX(i, j - 1)] - 4. * uk[IDX(i, j)]);
+    un[k] = v;
+  }
+  __syncthreads();   // every read of Uⁿ in this tile is done
+#pragma unroll
+  for (int k = 0; k < NEQ; ++k) ub[k * NC + IDX(i, j)] = un[k];
+  if (ghost) {
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) ub[k * NC + IDX(gi, gj)] = gh[k];
+  }
+  if (err) atomicOr(&S->err, WX_EHLLC);
+}
+
+// ---------------------------------------------------------------------
+// Block statistics: CFL partials (src/timestep.f90:29-41) and the
+// refinement flag (src/criteria.f90:16-50) from a tile of primitives.
+// stat[2l] = min dx/(|u|+cs), min dy/(|v|+cs) ; stat[2l+1] = DMC t_cool min.
+// flag: 1 refine, 2 coarsen, 0 neither.
+template <int NEQ>
+__device__ void block_stats(const wx_params *__restrict__ p, const double *prim, const LeafRec &L,
+                            double *stat, int8_t *flag, int l) {
+  __shared__ double red[3][4];
+  const int tid = threadIdx.x;
+  const int i = tid % 16 + 1, j = tid / 16 + 1;
+  const int lev = L.level;
+  double dtp = 1.e30, dtc = 1e30, grad = 0.;
+  {
+    const double rho = prim[0 * NC + IDX(i, j)], vx = prim[1 * NC + IDX(i, j)];
+    const double vy = prim[2 * NC + IDX(i, j)], pr = prim[3 * NC + IDX(i, j)];
+    const double cs = sqrt(p->gamma * pr / rho);
+    dtp = dmin(dtp, p->dx[lev - 1] / (fabs(vx) + cs));
+    dtp = dmin(dtp, p->dy[lev - 1] / (fabs(vy) + cs));
+    if (p->cooling == 2) {
+      double T = (pr / rho) * p->Tempsc;
+      if (T > 1e4) {
+        double al = cooldmc(p, T) * (rho * rho);
+        dtc = dmin(dtc, p->cv * pr / al);
+      }
+    }
+    const double am = dmax(1e-15, fabs(pr));
+    const double am2 = dmax(1e-15, fabs(rho));
+    const double *P4 = prim + 3 * NC, *P1 = prim;
+    grad = dmax(dmax(dmax(fabs(P4[IDX(i + 1, j)] - pr) / am, fabs(P4[IDX(i - 1, j)] - pr) / am),
+                     fabs(P4[IDX(i, j + 1)] - pr) / am), fabs(P4[IDX(i, j - 1)] - pr) / am);
+    grad = dmax(grad, dmax(dmax(dmax(fabs(P1[IDX(i + 1, j)] - rho) / am2, fabs(P1[IDX(i - 1, j)] - rho) / am2),
+                                fabs(P1[IDX(i, j + 1)] - rho) / am2), fabs(P1[IDX(i, j - 1)] - rho) / am2));
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    dtp = dmin(dtp, __shfl_xor(dtp, o));
+    dtc = dmin(dtc, __shfl_xor(dtc, o));
+    grad = dmax(grad, __shfl_xor(grad, o));
+  }
+  const int w = tid >> 6;
+  if ((tid & 63) == 0) { red[0][w] = dtp; red[1][w] = dtc; red[2][w] = grad; }
+  __syncthreads();
+  if (tid == 0) {
+    for (int k = 1; k < 4; ++k) {
+      dtp = dmin(dtp, red[0][k]);
+      dtc = dmin(dtc, red[1][k]);
+      grad = dmax(grad, red[2][k]);
+    }
+    stat[2 * l] = dtp;
+    stat[2 * l + 1] = dtc;
+    int8_t fl = 0;
+    if (grad >= p->tol) fl = (lev < p->nlevs) ? 1 : 0;
+    else if (grad <= p->tol2) fl = 2;
+    flag[l] = fl;
+  }
+}
+
+// ---------------------------------------------------------------------
+// boundaryI + cooling (+ CFL / criteria epilogue): src/tstep.f90:92-122.
+template <int NEQ>
+__global__ __launch_bounds__(256) void k_fillI_cool(const wx_params *__restrict__ p,
+                                                    const LeafRec *__restrict__ leaves,
+                                                    const int2 *__restrict__ ops,
+                                                    const BlockRec *__restrict__ binfo,
+                                                    const double *__restrict__ Uin,
+                                                    double *__restrict__ Uout,
+                                                    double *__restrict__ stat,
+                                                    int8_t *__restrict__ flag,
+                                                    DevScalars *__restrict__ S) {
+  __shared__ double tile[NEQ * NC];
+  const int l = blockIdx.x;
+  const LeafRec L = leaves[l];
+  const int b = L.id;
+  const int tid = threadIdx.x;
+  const double time = S->time;
+  const bool imp = (L.flags & 1) != 0;
+  const double vjet = p->otherb ? jet_velocity(p, time) : 0.0;
+  const double *ib = Uin + (size_t)b * NEQ * NC;
+  // carried tile (interior U^{n+1}, ring of U^{n+1/2}); impose where listed
+  for (int c = tid; c < NC; c += blockDim.x) {
+    double uu[NEQ];
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) uu[k] = ib[k * NC + c];
+    if (imp) {
+      const int i = c % NP - 1, j = c / NP - 1;
+      if (in_jet(p, L.level, L.icx, L.icy, i, j)) jet_state<NEQ>(p, vjet, uu);
+    }
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) tile[k * NC + c] = uu[k];
+  }
+  __syncthreads();
+  // ghost layer 1 from the list (64 cells)
+  if (tid < 64) {
+    const int side = tid >> 4, pos = (tid & 15) + 1;
+    const int2 op = ops[l * 8 + side * 2 + ((pos - 1) >> 3)];
+    if ((op.y & 15) != GK_NONE) {
+      double g[NEQ];
+      ghost_gather<NEQ>(p, Uin, binfo, op, b, L.level, L.icx, L.icy, side, 1, pos, vjet, g);
+      int gi, gj;
+      ghost_coords(side, 1, pos, gi, gj);
+#pragma unroll
+      for (int k = 0; k < NEQ; ++k) tile[k * NC + IDX(gi, gj)] = g[k];
+    }
+  }
+  __syncthreads();
+  // cooling over cells 0..n+1 (src/coolingh.f90:19-20, src/coolingdmc.f90:23-24)
+  const int mode = p->cooling;
+  if (mode != 0) {
+    const double dts = S->dt * p->tsc;
+    for (int c = tid; c < 18 * 18; c += blockDim.x) {
+      const int i = c % 18, j = c / 18;
+      double uu[NEQ];
+#pragma unroll
+      for (int k = 0; k < NEQ; ++k) uu[k] = tile[k * NC + IDX(i, j)];
+      if (mode == 1) atomic_h<NEQ>(p, dts, uu);
+      else cool_dmc<NEQ>(p, dts, uu);
+#pragma unroll
+      for (int k = 0; k < NEQ; ++k) tile[k * NC + IDX(i, j)] = uu[k];
+    }
+    __syncthreads();
+  }
+  double *ob = Uout + (size_t)b * NEQ * NC;
+  for (int c = tid; c < NEQ * NC; c += blockDim.x) ob[c] = tile[c];
+  __syncthreads();
+  // primitives of cells 0..17 (non-corner use only) for the epilogue
+  for (int c = tid; c < 18 * 18; c += blockDim.x) {
+    const int i = c % 18, j = c / 18;
+    double uu[NEQ], q[NEQ], T;
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) uu[k] = tile[k * NC + IDX(i, j)];
+    uprim<NEQ>(p, q, uu, T);
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) tile[k * NC + IDX(i, j)] = q[k];
+  }
+  __syncthreads();
+  block_stats<NEQ>(p, tile, L, stat, flag, l);
+}
+
+// Statistics only (after upload or a mesh change).
+template <int NEQ>
+__global__ __launch_bounds__(256) void k_blockstat(const wx_params *__restrict__ p,
+                                                   const LeafRec *__restrict__ leaves,
+                                                   const double *__restrict__ U,
+                                                   double *__restrict__ stat,
+                                                   int8_t *__restrict__ flag) {
+  __shared__ double tile[NEQ * NC];
+  const int l = blockIdx.x;
+  const LeafRec L = leaves[l];
+  const double *ub = U + (size_t)L.id * NEQ * NC;
+  for (int c = threadIdx.x; c < 18 * 18; c += blockDim.x) {
+    const int i = c % 18, j = c / 18;
+    double uu[NEQ], q[NEQ], T;
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) uu[k] = ub[k * NC + IDX(i, j)];
+    uprim<NEQ>(p, q, uu, T);
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) tile[k * NC + IDX(i, j)] = q[k];
+  }
+  __syncthreads();
+  block_stats<NEQ>(p, tile, L, stat, flag, l);
+}
+
+// dt = min(cfl*dtp, 0.4*dtcool*Psc/tsc) (src/timestep.f90:50); optionally
+// time += dt (src/main.f90:88).  One workgroup.
+__global__ __launch_bounds__(1024) void k_dtreduce(const wx_params *__restrict__ p,
+                                                   const double *__restrict__ stat, int nleaf,
+                                                   DevScalars *__restrict__ S, int advance) {
+  __shared__ double r0[16], r1[16];
+  double a = 1.e30, c = 1e30;
+  for (int l = threadIdx.x; l < nleaf; l += blockDim.x) {
+    a = dmin(a, stat[2 * l]);
+    c = dmin(c, stat[2 * l + 1]);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a = dmin(a, __shfl_xor(a, o));
+    c = dmin(c, __shfl_xor(c, o));
+  }
+  if ((threadIdx.x & 63) == 0) { r0[threadIdx.x >> 6] = a; r1[threadIdx.x >> 6] = c; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int k = 1; k < nw; ++k) { a = dmin(a, r0[k]); c = dmin(c, r1[k]); }
+    S->dtp_min = a;
+    S->dtcool_min = c;
+    double dt = dmin(p->cfl * a, 0.4 * c * p->Psc / p->tsc);
+    S->dt = dt;
+    if (advance) S->time = S->time + dt;
+  }
+}
+
+// ---------------------------------------------------------------------
+// admesh data ops: src/refine.f90:30-42 (injection over the padded block)
+// and src/coarse.f90:182-366 (MPIP branch: 2x2 means incl. ghosts, then
+// outer ghost copies).  One workgroup per op.
+template <int NEQ>
+__global__ __launch_bounds__(256) void k_meshops(const MeshOp *__restrict__ ops, double *__restrict__ U) {
+  const MeshOp op = ops[blockIdx.x];
+  const int tid = threadIdx.x;
+  if (op.kind == 0) {
+    const double *pb = U + (size_t)op.a * NEQ * NC;
+    const int sons[4] = {op.s1, op.s2, op.s3, op.s4};
+    for (int c = tid; c < 4 * NC; c += blockDim.x) {
+      const int s = c / NC, cc = c % NC;
+      const int i = cc % NP - 1, j = cc / NP - 1;
+      const int si = (i + 1) / 2 + ((s & 1) ? NX / 2 : 0);
+      const int sj = (j + 1) / 2 + ((s & 2) ? NY / 2 : 0);
+      double *sb = U + (size_t)sons[s] * NEQ * NC;
+#pragma unroll
+      for (int k = 0; k < NEQ; ++k) sb[k * NC + cc] = pb[k * NC + IDX(si, sj)];
+    }
+  } else {
+    __shared__ double d[NEQ * NC];
+    const int sons[4] = {op.s1, op.s2, op.s3, op.s4};
+    // start from the current dad tile (corners the rules below do not set)
+    const double *dbg = U + (size_t)op.a * NEQ * NC;
+    for (int c = tid; c < NEQ * NC; c += blockDim.x) d[c] = dbg[c];
+    __syncthreads();
+    // dad cells (0..17)^2 from the four sons: son s covers i in [is0, is1]
+    for (int c = tid; c < 18 * 18; c += blockDim.x) {
+      const int i = c % 18, j = c / 18;
+      const int sx = (i <= NX / 2) ? 0 : 1, sy = (j <= NY / 2) ? 0 : 1;
+      const int s = sx + 2 * sy;
+      const int li = i - sx * (NX / 2), lj = j - sy * (NY / 2);
+      const double *sb = U + (size_t)sons[s] * NEQ * NC;
+      const int i1 = 2 * li - 1, j1 = 2 * lj - 1;
+#pragma unroll
+      for (int k = 0; k < NEQ; ++k)
+        d[k * NC + IDX(i, j)] = 0.25 * (((sb[k * NC + IDX(i1, j1)] + sb[k * NC + IDX(i1 + 1, j1)])
+                                        + sb[k * NC + IDX(i1, j1 + 1)]) + sb[k * NC + IDX(i1 + 1, j1 + 1)]);
+    }
+    __syncthreads();
+    // outer ghost copies (src/coarse.f90:349-358), same order
+    if (tid < NEQ) {
+      double *dk = d + tid * NC;
+      for (int j = -1; j <= NY + 2; ++j) dk[IDX(-1, j)] = dk[IDX(0, j)];
+      for (int j = -1; j <= NY + 2; ++j) dk[IDX(NX + 2, j)] = dk[IDX(NX + 1, j)];
+      for (int i = -1; i <= NX + 2; ++i) dk[IDX(i, -1)] = dk[IDX(i, 0)];
+      for (int i = -1; i <= NX + 2; ++i) dk[IDX(i, NY + 2)] = dk[IDX(i, NY + 1)];
+      dk[IDX(-1, -1)] = dk[IDX(0, -1)];
+      dk[IDX(NX + 2, -1)] = dk[IDX(NX + 2, 0)];
+      dk[IDX(-1, NY + 2)] = dk[IDX(-1, NY + 1)];
+      dk[IDX(NX + 2, NY + 2)] = dk[IDX(NX + 1, NY + 2)];
+    }
+    __syncthreads();
+    double *db = U + (size_t)op.a * NEQ * NC;
+    for (int c = tid; c < NEQ * NC; c += blockDim.x) db[c] = d[c];
+  }
+}
+
+// Layout transposes between [id][var][j][i] and the reference's
+// u(nb, ieq, i, j) with nb fastest (ld = number of block rows).
+template <int NEQ>
+__global__ void k_transpose(int cap, const double *__restrict__ src, double *__restrict__ dst,
+                            int to_fortran, int ld) {
+  const size_t n = (size_t)cap * NEQ * NC;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    if (to_fortran) {
+      // t enumerates the Fortran array: nb fastest
+      const size_t nb = t % cap, r = t / cap;
+      const size_t k = r % NEQ, cell = r / NEQ;     // cell = (i+1) + 20*(j+1)
+      dst[t] = src[((nb + 1) * NEQ + k) * NC + cell];
+    } else {
+      const size_t cell = t % NC, r = t / NC;
+      const size_t k = r % NEQ, id = r / NEQ;       // id 0..cap-1 -> block id+1
+      dst[((id + 1) * NEQ + k) * NC + cell] = src[(cell * NEQ + k) * (size_t)ld + id];
+    }
+  }
+}
+
+template <int NEQ>
+__global__ void k_calcprim(const wx_params *__restrict__ p, int cap, const double *__restrict__ U,
+                           double *__restrict__ prim) {
+  const size_t n = (size_t)(cap + 1) * NC;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const size_t id = t / NC, c = t % NC;
+    double uu[NEQ], q[NEQ], T;
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) uu[k] = U[(id * NEQ + k) * NC + c];
+    if (uu[0] == 0.0 && uu[3] == 0.0) {
+#pragma unroll
+      for (int k = 0; k < NEQ; ++k) q[k] = 0.0;
+    } else {
+      uprim<NEQ>(p, q, uu, T);
+    }
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) prim[(id * NEQ + k) * NC + c] = q[k];
+  }
+}
+
+// ---------------------------------------------------------------------
+// launch wrappers
+#define WX_DISPATCH(NEQV, CALL)                  \
+  switch (NEQV) {                                \
+    case 4: { constexpr int N = 4; CALL; } break; \
+    case 5: { constexpr int N = 5; CALL; } break; \
+    case 6: { constexpr int N = 6; CALL; } break; \
+    default: return WX_EARG;                     \
+  }
+
+static inline int hip_status() { return hipGetLastError() == hipSuccess ? WX_OK : WX_EHIP; }
+
+int launch_stageA(const wx_params *dP, int neq, const LeafRec *leaves, int nleaf, const double *U,
+                  double *Q, DevScalars *S, void *stream) {
+  if (nleaf <= 0) return WX_OK;
+  WX_DISPATCH(neq, hipLaunchKernelGGL(k_stageA<N>, dim3(nleaf), dim3(256), 0, (hipStream_t)stream, dP, leaves, U, Q, S));
+  return hip_status();
+}
+
+int launch_stageB(const wx_params *dP, int neq, const LeafRec *leaves, int nleaf, const int2 *ops,
+                  const BlockRec *binfo, double *U, const double *Q, DevScalars *S, void *stream) {
+  if (nleaf <= 0) return WX_OK;
+  WX_DISPATCH(neq, hipLaunchKernelGGL(k_stageB<N>, dim3(nleaf), dim3(256), 0, (hipStream_t)stream, dP, leaves, ops, binfo, U, Q, S));
+  return hip_status();
+}
+
+int launch_fillI_cool(const wx_params *dP, int neq, const LeafRec *leaves, int nleaf, const int2 *ops,
+                      const BlockRec *binfo, const double *Uin, double *Uout, double *stat,
+                      int8_t *flag, DevScalars *S, void *stream) {
+  if (nleaf <= 0) return WX_OK;
+  WX_DISPATCH(neq, hipLaunchKernelGGL(k_fillI_cool<N>, dim3(nleaf), dim3(256), 0, (hipStream_t)stream, dP, leaves, ops, binfo, Uin, Uout, stat, flag, S));
+  return hip_status();
+}
+
+int launch_blockstat(const wx_params *dP, int neq, const LeafRec *leaves, int nleaf, const double *U,
+                     double *stat, int8_t *flag, void *stream) {
+  if (nleaf <= 0) return WX_OK;
+  WX_DISPATCH(neq, hipLaunchKernelGGL(k_blockstat<N>, dim3(nleaf), dim3(256), 0, (hipStream_t)stream, dP, leaves, U, stat, flag));
+  return hip_status();
+}
+
+int launch_dtreduce(const wx_params *dP, const double *stat, int nleaf, DevScalars *S, int advance,
+                    void *stream) {
+  hipLaunchKernelGGL(k_dtreduce, dim3(1), dim3(1024), 0, (hipStream_t)stream, dP, stat, nleaf, S, advance);
+  return hip_status();
+}
+
+int launch_meshops(const wx_params *dP, int neq, const MeshOp *ops, int nops, double *U, void *stream) {
+  (void)dP;
+  if (nops <= 0) return WX_OK;
+  WX_DISPATCH(neq, hipLaunchKernelGGL(k_meshops<N>, dim3(nops), dim3(256), 0, (hipStream_t)stream, ops, U));
+  return hip_status();
+}
+
+int launch_transpose(int neq, int cap, const double *src, double *dst, int to_fortran, int ld, void *stream) {
+  WX_DISPATCH(neq, hipLaunchKernelGGL(k_transpose<N>, dim3(1024), dim3(256), 0, (hipStream_t)stream, cap, src, dst, to_fortran, ld));
+  return hip_status();
+}
+
+int launch_calcprim(const wx_params *dP, int neq, int cap, const double *U, double *prim, void *stream) {
+  WX_DISPATCH(neq, hipLaunchKernelGGL(k_calcprim<N>, dim3(1024), dim3(256), 0, (hipStream_t)stream, dP, cap, U, prim));
+  return hip_status();
+}
+
+}  // namespace wx

@@ -1,0 +1,347 @@
+// wx_physics.h -- per-cell / per-face FP64 physics for the gfx950 kernels.
+//
+// Every function keeps the reference's evaluation order term by term (the
+// library is compiled with -ffp-contract=off), so results agree with the
+// Fortran to the last few ulps; the only differences come from the device
+// libm (exp/log/pow/sin) against glibc.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "wx_device.h"
+
+namespace wx {
+
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+
+// src/uprim.f90:5-63 : conserved -> primitive, floors, T by cooling mode.
+template <int NEQ>
+__device__ __forceinline__ void uprim(const wx_params *__restrict__ p, double *q,
+                                      const double *uu, double &T) {
+  const double cv = p->cv;
+  double r = dmax(uu[0], 1e-15);
+  q[0] = r;
+  q[1] = uu[1] / r;
+  q[2] = uu[2] / r;
+#pragma unroll
+  for (int k = 4; k < NEQ; ++k) q[k] = uu[k];
+  q[3] = (uu[3] - 0.5 * r * (q[1] * q[1] + q[2] * q[2])) / cv;
+  q[3] = dmax(q[3], 1e-15);
+  const int mode = p->cooling;
+  if (mode == 0) {
+    T = q[3] / r;
+  } else if (mode == 1) {
+    double dentot = (2.0 * r - (NEQ > 4 ? q[NEQ > 4 ? 4 : 0] : 0.0));
+    dentot = dmax(dentot, 1e-15);
+    T = dmax(1.0, (q[3] / dentot) * p->Tempsc);
+    q[3] = dentot * T / p->Tempsc;
+  } else {
+    r = dmax(r, 1e-15);
+    T = dmax(1.0, (q[3] / (r)) * p->Tempsc);
+    q[3] = r * T / p->Tempsc;
+  }
+}
+
+// src/primu.f90:4-19
+template <int NEQ>
+__device__ __forceinline__ void primu(double cv, const double *q, double *uu) {
+  uu[0] = q[0];
+  uu[1] = q[0] * q[1];
+  uu[2] = q[0] * q[2];
+  uu[3] = 0.5 * q[0] * (q[1] * q[1] + q[2] * q[2]) + cv * q[3];
+#pragma unroll
+  for (int k = 4; k < NEQ; ++k) uu[k] = q[k];
+}
+
+// src/primf.f90:5-23
+template <int NEQ>
+__device__ __forceinline__ void primf(double cv, const double *q, double *ff) {
+  double etot = 0.5 * q[0] * (q[1] * q[1] + q[2] * q[2]) + cv * q[3];
+  ff[0] = q[0] * q[1];
+  ff[1] = q[0] * q[1] * q[1] + q[3];
+  ff[2] = q[0] * q[1] * q[2];
+  ff[3] = q[1] * (etot + q[3]);
+#pragma unroll
+  for (int k = 4; k < NEQ; ++k) ff[k] = q[k] * q[1];
+}
+
+// src/limiter.f90:26-87 `average`
+__device__ __forceinline__ double average(int lim, double a, double b) {
+  double s, c, d, r;
+  switch (lim) {
+    case -1: return 0.0;
+    case 0: return 0.5 * (a + b);
+    case 1:
+      s = copysign(1.0, a);
+      return s * dmax(0.0, dmin(fabs(a), s * b));
+    case 2:
+      if (a * b <= 0.0) return 0.0;
+      return a * b * (a + b) / (a * a + b * b);
+    case 3: {
+      const double delta = 1.e-7;
+      return (a * (b * b + delta) + b * (a * a + delta)) / (a * a + b * b + delta);
+    }
+    case 4:
+      s = copysign(1.0, a);
+      c = 0.25 * a + 0.75 * b;
+      d = 0.75 * a + 0.25 * b;
+      r = dmin(dmin(dmin(2.0 * fabs(a), 2.0 * s * b), s * c), s * d);
+      return s * dmax(0.0, r);
+    case 5:
+      s = copysign(1.0, a);
+      c = 0.5 * (a + b);
+      r = dmin(dmin(2.0 * fabs(a), 2.0 * s * b), s * c);
+      return s * dmax(0.0, r);
+    case 6: {
+      s = copysign(1.0, b);
+      double av1 = dmin(2.0 * fabs(b), s * a);
+      double av2 = dmin(fabs(b), 2.0 * s * a);
+      return s * dmax(dmax(0.0, av1), av2);
+    }
+    default: return 0.0;
+  }
+}
+
+// src/limiter.f90:7-24 (all variables, passives included)
+template <int NEQ>
+__device__ __forceinline__ void limiter(int lim, const double *pll, double *pl, double *pr,
+                                        const double *prr) {
+#pragma unroll
+  for (int k = 0; k < NEQ; ++k) {
+    double dl = pl[k] - pll[k];
+    double dm = pr[k] - pl[k];
+    double dr = prr[k] - pr[k];
+    double al = average(lim, dl, dm);
+    double ar = average(lim, dm, dr);
+    pl[k] = pl[k] + al * 0.5;
+    pr[k] = pr[k] - ar * 0.5;
+  }
+}
+
+// HLLC star flux on one side: src/hllcfluxes.f90:131-166.
+template <int NEQ>
+__device__ __forceinline__ void hllc_star(const wx_params *__restrict__ p, const double *q,
+                                          double s, double smu, double sst, bool right,
+                                          double *ff) {
+  const double cv = p->cv;
+  double uu[NEQ], uuk[NEQ];
+  double rhost = q[0] * (smu) / (s - sst);
+  double ek = 0.5 * q[0] * (q[1] * q[1] + q[2] * q[2]) + cv * q[3];
+  uuk[0] = rhost;
+  uuk[1] = rhost * sst;
+  uuk[2] = rhost * q[2];
+  double den = (right && (p->compat & 1)) ? (q[3] * smu) : (q[0] * smu);
+  uuk[3] = rhost * (ek / q[0] + (sst - q[1]) * (sst + q[3] / den));
+#pragma unroll
+  for (int k = 4; k < NEQ; ++k) uuk[k] = rhost * q[k] / q[0];
+  primf<NEQ>(cv, q, ff);
+  primu<NEQ>(cv, q, uu);
+#pragma unroll
+  for (int k = 0; k < NEQ; ++k) ff[k] = ff[k] + s * (uuk[k] - uu[k]);
+}
+
+// Riemann solvers: HLLC (src/hllcfluxes.f90:98-173), HLL
+// (src/hllfluxes.f90:98-131), HLL/HLLC hybrid (src/hll_hllcfluxes.f90:95-209).
+// Returns nonzero on the reference's `stop` path (NaN star speed).
+template <int NEQ>
+__device__ __forceinline__ int riemann(const wx_params *__restrict__ p, const double *pl,
+                                       const double *pr, double *ff) {
+  const double cv = p->cv, gam = p->gamma;
+  const int solver = p->solver;
+  double beta1 = 1.0, beta2 = 0.0;
+  if (solver == 2) {
+    double dq1 = pr[1] - pl[1], dq2 = pr[2] - pl[2];
+    double mag = sqrt(dq1 * dq1 + dq2 * dq2);
+    if (mag <= 1e-6) {
+      beta1 = 1.0;
+      beta2 = 0.0;
+    } else {
+      double alpha1 = fabs(dq1) / mag;
+      beta1 = sqrt(alpha1);
+      beta2 = 1.0 - beta1;
+    }
+  }
+  double csl = sqrt(gam * pl[3] / pl[0]);
+  double csr = sqrt(gam * pr[3] / pr[0]);
+  double sr = dmax(pl[1] + csl, pr[1] + csr);
+  double sl = dmin(pl[1] - csl, pr[1] - csr);
+  if (sl > 0) { primf<NEQ>(cv, pl, ff); return 0; }
+  if (sr < 0) { primf<NEQ>(cv, pr, ff); return 0; }
+  if (solver == 1) {
+    double fl[NEQ], fr[NEQ], ul[NEQ], ur[NEQ];
+    primf<NEQ>(cv, pl, fl); primf<NEQ>(cv, pr, fr);
+    primu<NEQ>(cv, pl, ul); primu<NEQ>(cv, pr, ur);
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k)
+      ff[k] = (sr * fl[k] - sl * fr[k] + sl * sr * (ur[k] - ul[k])) / (sr - sl);
+    return 0;
+  }
+  double slmul = sl - pl[1];
+  double srmur = sr - pr[1];
+  double rholul = pl[0] * pl[1];
+  double rhorur = pr[0] * pr[1];
+  double sst = (srmur * rhorur - slmul * rholul - pr[3] + pl[3]) / (srmur * pr[0] - slmul * pl[0]);
+  int err = 0;
+  if (sst >= 0.0) hllc_star<NEQ>(p, pl, sl, slmul, sst, false, ff);
+  else if (sst <= 0.0) hllc_star<NEQ>(p, pr, sr, srmur, sst, true, ff);
+  else {
+    err = 1;
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) ff[k] = 0.0;
+  }
+  if (solver == 2 && !err) {
+    double fl[NEQ], fr[NEQ], ul[NEQ], ur[NEQ];
+    primf<NEQ>(cv, pl, fl); primf<NEQ>(cv, pr, fr);
+    primu<NEQ>(cv, pl, ul); primu<NEQ>(cv, pr, ur);
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k)
+      ff[k] = beta1 * (ff[k]) + beta2 * ((sr * fl[k] - sl * fr[k] + sl * sr * (ur[k] - ul[k])) / (sr - sl));
+  }
+  return err;
+}
+
+// ---------------------------------------------------------------------
+// Cooling.  src/coolingh.f90:41-76 rates, :88-165 ALOSS, :171-259 atomic.
+__device__ __forceinline__ double rate_alpha(double T) { return 2.55e-13 * pow(1.e4 / T, 0.79); }
+__device__ __forceinline__ double rate_colf(double T) { return 5.83e-11 * sqrt(T) * exp(-(157828. / T)); }
+__device__ __forceinline__ double rate_betah(double T) {
+  double a = 157890. / T;
+  return 1.133e-24 / sqrt(a) * (-0.0713 + 0.5 * log(a) + 0.640 * pow(a, -0.33333));
+}
+
+__device__ __forceinline__ double aloss(double x1, double den, double dh0, double te0) {
+  const double XION = 2.179e-11, XO = 1.e-3;
+  const double C0 = 0.5732, C1 = 1.8288e-5, C2 = -1.15822e-10, C3 = 9.4288e-16;
+  const double D0 = 0.5856, D1 = 1.55083e-5, D2 = -9.669e-12, D3 = 5.716e-19;
+  const double ENK = 118409., EN = 1.634E-11;
+  double TE = dmax(te0, 1000.);
+  double DH = den;
+  double DHP = (1. - x1) * DH;
+  double DE = DHP + 1.E-4 * DH;
+  double DOI = XO * dh0;
+  double DOII = XO * DHP;
+  double OMEGA = 0.0;
+  if (TE <= 55000.) OMEGA = C0 + TE * (C1 + TE * (C2 + TE * C3));
+  if (TE >= 72000.) OMEGA = D0 + TE * (D1 + TE * (D2 + TE * D3));
+  if (TE > 55000. && TE < 72000.) {
+    double OMEGAL = C0 + TE * (C1 + TE * (C2 + TE * C3));
+    double OMEGAH = D0 + TE * (D1 + TE * (D2 + TE * D3));
+    double FRAC = (TE - 55000.) / 17000.;
+    OMEGA = (1. - FRAC) * OMEGAL + FRAC * OMEGAH;
+  }
+  double QLA = 8.6287E-6 / (2. * sqrt(TE)) * OMEGA * exp(-(ENK / TE));
+  double ECOLL = DE * dh0 * QLA * EN;
+  ECOLL = dmax(ECOLL, 0.);
+  double CION = 5.834E-11 * sqrt(TE) * exp(-(1.579E5 / TE));
+  double EION = DE * dh0 * CION * XION;
+  double EREC = DE * DHP * (rate_betah(TE));
+  EREC = dmax(EREC, 0.);
+  double TM = 1. / TE;
+  double T2 = TM * TM;
+  double EOI = DE * DOI * pow(10., 1381465 * T2 - 12328.69 * TM - 19.82621);
+  double EOII = DE * DOII * pow(10., -(2061075. * T2) - 14596.24 * TM - 19.01402);
+  EOI = dmax(EOI, 0.);
+  EOII = dmax(EOII, 0.);
+  double BETAF = 1.3 * 1.42E-27 * pow(TE, 0.5);
+  double HIICOOL = DE * DHP * BETAF;
+  double EQUIL = (1.0455E-18 / pow(TE, 0.63)) * (1. - exp(-pow(TE * 1.E-5, 1.63))) * DE * den + HIICOOL;
+  double FR = 0.0;
+  if (TE <= 44770.) FR = 0.;
+  if (TE >= 54770.) FR = 1.;
+  if (TE > 44770. && TE < 54770.) {
+    double EX2 = exp(-(2. * (TE - 49770.) / 500.));
+    double TANH = (1. - EX2) / (1. + EX2);
+    FR = 0.5 * (1. + TANH);
+  }
+  return ECOLL + EION + (EREC + 7.033 * (EOI + EOII)) * (1. - FR) + EQUIL * FR;
+}
+
+// src/coolingh.f90:171-259 atomic: H neutral fraction + radiative loss.
+template <int NEQ>
+__device__ __forceinline__ void atomic_h(const wx_params *__restrict__ p, double dt, double *uu) {
+  if constexpr (NEQ >= 5) {
+    const double xi = 1.e-4, boltzm = 1.3807e-16;
+    double q[NEQ], T;
+    uprim<NEQ>(p, q, uu, T);
+    double col = rate_colf(T);
+    double rec = rate_alpha(T);
+    double y0 = uu[4] / uu[0];
+    double dh = uu[0];
+    double a = rec + col;
+    double b = -((2. + xi) * rec + (1. + xi) * col);
+    double c = (1. + xi) * rec;
+    double d = sqrt(b * b - 4. * a * c);
+    double g0 = (2. * a * y0 + b + d) / (2. * a * y0 + b - d);
+    double e = exp(-(d * dh * dt));
+    double y1 = (-b - d * (1. + g0 * e) / (1. - g0 * e)) / (2. * a);
+    y1 = dmin(y1, 1.);
+    y1 = dmax(y1, 0.0001);
+    uu[4] = y1 * uu[0];
+    double dh0 = uu[4];
+    double al = aloss(y0, dh, dh0, T) / (dh * dh);
+    double tprime = 10.;
+    double ce = (2. * dh * al) / (3. * boltzm * T);
+    double t1 = tprime + (T - tprime) * exp(-(ce * dt));
+    t1 = dmax(t1, 0.1 * T);
+    t1 = dmin(t1, 10. * T);
+    t1 = dmax(t1, tprime);
+    uu[3] = p->cv * (2. * uu[0] - uu[4]) * t1 / p->Tempsc + 0.5 * q[0] * (q[1] * q[1] + q[2] * q[2]);
+  }
+}
+
+// src/dmc_module.f90:14-32 cooldmc (row 42 at T == 1e8 clamps to row 41)
+__device__ __forceinline__ double cooldmc(const wx_params *__restrict__ p, double T) {
+  if (T > 1e8) return 0.21e-26 * sqrt(T);
+  int if1 = (int)(log10(T) * 10) - 39;
+  int if2 = if1 + 1;
+  if (if1 < 1) if1 = 1;
+  if (if2 > 41) if2 = 41;
+  double T0 = p->cooltab[2 * (if1 - 1)], c0 = p->cooltab[2 * (if1 - 1) + 1];
+  double T1 = p->cooltab[2 * (if2 - 1)], c1 = p->cooltab[2 * (if2 - 1) + 1];
+  if (T1 == T0) return c0;
+  return (c1 - c0) * (T - T0) / (T1 - T0) + c0;
+}
+
+// src/coolingdmc.f90:19-50 on one cell's conserved state
+template <int NEQ>
+__device__ __forceinline__ void cool_dmc(const wx_params *__restrict__ p, double dt, double *uu) {
+  double q[NEQ], T;
+  uprim<NEQ>(p, q, uu, T);
+  if (T > 1.e4) {
+    double Eth0 = p->cv * q[3];
+    double Aloss = cooldmc(p, T);
+    double dens = q[0];
+    double Ce = (Aloss * (dens * dens)) / (Eth0 * p->Psc);
+    double pn = q[3] * exp(-(Ce * dt));
+    uu[3] = uu[3] - Eth0 + p->cv * pn;
+  }
+}
+
+// Jet source (JET/jet.f90:41-77, through src/user_mod.f90:73-84).
+struct JetState {
+  double vjet;
+};
+
+__device__ __forceinline__ bool in_jet(const wx_params *__restrict__ p, int level, int icx,
+                                       int icy, int i, int j) {
+  double x = ((double)(i + icx) - 0.5) * p->dx[level - 1];
+  double y = ((double)(j + icy) - 0.5) * p->dy[level - 1];
+  return (fabs(x - p->jet[2]) <= p->jet[1]) && (fabs(y - p->jet[3]) <= p->jet[0]);
+}
+
+template <int NEQ>
+__device__ __forceinline__ void jet_state(const wx_params *__restrict__ p, double vjet, double *uu) {
+  const double den = p->jet[4];
+  uu[0] = den;
+  uu[1] = den * vjet;
+  uu[2] = 0.;
+  uu[3] = 0.5 * den * (vjet * vjet) + p->cv * den * p->jet[10] * p->jet[5];
+  if constexpr (NEQ >= 5) uu[4] = den * p->jet[9];
+  if constexpr (NEQ >= 6) uu[5] = den;
+}
+
+__device__ __forceinline__ double jet_velocity(const wx_params *__restrict__ p, double time) {
+  const double pi = 3.141592653589793;  // acos(-1.)
+  return p->jet[6] + p->jet[7] * sin(2. * pi * time / p->jet[8]);
+}
+
+}  // namespace wx
