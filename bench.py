@@ -1,0 +1,148 @@
+#!/usr/bin/env python3
+"""Throughput of the plain FP64 hot path on one MI355X, in zone-updates/s.
+
+A step is one iteration of the reference's time loop (src/main.f90:87-98):
+CFL timestep, time += dt, tstep (stage A, ghost fill II, stage B + viscosity,
+ghost fill I, cooling) and admesh, over every leaf block of the mesh, which
+stays resident in HBM.  A zone-update is one interior leaf cell advanced by
+one step (ghost cells not counted).  The timed region is K steps after W
+warm-up steps, bracketed by device synchronisation.
+
+Prints one JSON line (see the keys below).  Options:
+  --gpus N          only N = 1 is supported (one GPU holds every leaf block)
+  --steps K --warmup W
+  --workload NAME   c3 (default: the variable cooling jet), c2, c1
+  --no-cpu-baseline skip the CPU oracle timing
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "zone-updates/s, 1 MI355X: plain FP64 vs batched+surrogate at fixed L1 error"
+
+# Algorithmic HBM bytes per zone-update of each kernel (neq doubles per cell):
+#   stage A      read U^n, write U^{n+1/2}                       2*neq*8
+#   stage B      read U^{n+1/2}, read U^n, write U^{n+1}         3*neq*8
+#   fillI_cool   read U^{n+1}, write the cooled U^{n+1}          2*neq*8
+# (the whole step's compulsory traffic is 5*neq*8: stages A and B).
+ALG_PASSES = {"stageA": 2, "stageB": 3, "fillI_cool": 2}
+
+
+def cpu_baseline(cfg, seconds: float = 12.0):
+    """The C oracle (single thread) on a bounded sample of the same workload:
+    same initial mesh, same steps (incl. admesh), until ~`seconds` of CPU."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from wxoracle import MeshState, Oracle
+    from walicxe2d_amd import params as P
+    O = Oracle()
+    pp = cfg.params()
+    m = MeshState(cfg.nblocks, cfg.neq)
+    O.basegrid(pp, m)
+    u, lp, ic = P.initial_state(cfg)
+    n = lp.shape[0] - 1
+    m.u[1:n + 1] = u[1:]
+    for nb in range(1, n + 1):
+        m.primit[nb] = O.calcprim(pp, m.u[nb])
+    if cfg.nlevs > 1:
+        for _ in range(4):
+            O.admesh(pp, m)
+    t = 0.0
+    zones = 0.0
+    steps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        t, _, z, err = O.run(pp, m, 1, t, do_admesh=cfg.nlevs > 1)
+        if err:
+            break
+        zones += z
+        steps += 1
+    el = time.perf_counter() - t0
+    return {"value": zones / el, "unit": "zone-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} steps of the same workload from t=0 ({zones:.0f} zone-updates, "
+                      f"{el:.1f} s), C oracle built with gcc -O2, one thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="c3")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported: this build runs every leaf block on one "
+              "GPU (multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from walicxe2d_amd import params as P
+    from walicxe2d_amd.capi import Simulation
+
+    cfg = P.CONFIGS[args.workload]()
+    sim = Simulation(cfg)
+    sim.basegrid()
+    amr = cfg.nlevs > 1
+    if amr:
+        for _ in range(4):            # src/main.f90:60-63
+            sim.admesh()
+    if args.warmup:
+        sim.step(args.warmup, admesh=amr)
+    sim.profile(1)                    # reset + record kernel events from here
+    sim.step(0)                       # synchronise
+    t0 = time.perf_counter()
+    _, _, zones = sim.step(args.steps, admesh=amr)   # returns after a device sync
+    el = time.perf_counter() - t0
+    prof = sim.profile(0)
+    nbmax, nleaf, _ = sim.info()
+    value = zones / el
+
+    neq = cfg.neq
+    dom = max(ALG_PASSES, key=lambda k: prof[k][0])
+    ms, launches = prof[dom]
+    alg_bytes = ALG_PASSES[dom] * neq * 8 * zones         # over all launches of the kernel
+    achieved = alg_bytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh).get(f"{dom}:{args.workload}")
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": dom, "kernel_ms_per_launch": round(ms / max(launches, 1), 5),
+            "alg_bytes_per_launch": round(alg_bytes / max(launches, 1)),
+            "step_frac": round(value * 5 * neq * 8 / (HBM_PEAK_GBS * 1e9), 4),
+            "kernels_ms": {k: round(v[0], 3) for k, v in prof.items() if v[1]}}
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / max(args.steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": cfg.name, "path": "plain-fp64", "roots": [cfg.nbrootx, cfg.nbrooty],
+                   "nlevs": cfg.nlevs, "neq": neq, "cooling": cfg.cooling, "leaf_blocks_end": nleaf,
+                   "block": "16x16+2 ghosts", "admesh_every_step": amr},
+        "roofline": roof,
+    }
+    sim.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(P.CONFIGS[args.workload]())
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

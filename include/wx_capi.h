@@ -138,6 +138,14 @@ int wx_step(wx_ctx *ctx, int32_t nsteps, int32_t do_admesh, double *time, double
             double *zone_updates);
 int wx_set_time(wx_ctx *ctx, double time);
 
+/* Per-kernel timing with HIP events recorded on the context's stream around
+ * every launch (no counterpart in the reference, which has no timers).
+ * Returns the accumulated milliseconds / launch counts of kernels
+ * 0 stage A, 1 stage B, 2 boundaryI+cooling, 3 dt reduction, 4 block
+ * statistics, 5 refine/coarse ops (n entries); then, if enable >= 0, resets
+ * the counters and switches recording on (1) or off (0). */
+int wx_profile(wx_ctx *ctx, int32_t enable, double *ms, int64_t *counts, int32_t n);
+
 /* Bookkeeping */
 int wx_info(wx_ctx *ctx, int32_t *nbmax, int32_t *nbleafs, int32_t *ninternal);
 int wx_last_error(char *buf, int32_t len);

@@ -38,16 +38,33 @@ def oracle_mesh(O: Oracle, cfg: P.Config, cap: int):
     return pp, m
 
 
-def rel_err(a: np.ndarray, b: np.ndarray, mask=MASK, floor_rel=1e-8):
-    """max over masked cells of |a-b| / max(|b|, floor_rel * max|b_var|), per variable."""
-    worst = 0.0
-    where = None
-    for k in range(a.shape[1]):
-        ak = a[:, k][:, mask]
-        bk = b[:, k][:, mask]
-        scale = max(np.max(np.abs(bk)), 1e-300)
-        d = np.abs(ak - bk) / np.maximum(np.abs(bk), floor_rel * scale)
-        if d.size and d.max() > worst:
-            worst = float(d.max())
-            where = (k, np.unravel_index(int(np.argmax(d)), d.shape))
-    return worst, where
+def cell_scale(b: np.ndarray, gamma: float = 5. / 3.) -> np.ndarray:
+    """Per-cell magnitude of each conserved variable, [blocks][neq][20][20]:
+    rho for density and scalars, rho*(|v| + c_s) for the momenta (a momentum
+    component can cross zero while the cell's momentum scale does not), and
+    the total energy for energy."""
+    rho = np.maximum(np.abs(b[:, 0]), 1e-300)
+    v2 = (b[:, 1] ** 2 + b[:, 2] ** 2) / rho ** 2
+    eint = np.maximum(b[:, 3] - 0.5 * rho * v2, 1e-300)
+    cs = np.sqrt(np.maximum(gamma * (gamma - 1.) * eint / rho, 0.0))
+    sc = np.empty_like(b)
+    sc[:, 0] = rho
+    sc[:, 1] = rho * (np.sqrt(v2) + cs)
+    sc[:, 2] = sc[:, 1]
+    sc[:, 3] = np.abs(b[:, 3])
+    for k in range(4, b.shape[1]):
+        sc[:, k] = rho
+    return sc
+
+
+def rel_err(a: np.ndarray, b: np.ndarray, mask=MASK):
+    """max over masked cells and variables of |a-b| / scale(cell, var), where
+    the scale is the cell's own magnitude of that variable (cell_scale)."""
+    sc = cell_scale(b)
+    d = np.abs(a - b) / np.maximum(sc, 1e-300)
+    d = d[:, :, mask]
+    if d.size == 0:
+        return 0.0, None
+    i = int(np.argmax(d))
+    idx = np.unravel_index(i, d.shape)
+    return float(d.max()), (int(idx[1]), int(idx[0]), int(idx[2]))

@@ -11,6 +11,11 @@ from harness import MASK, MASK_INTERIOR, oracle_mesh, rel_err
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-12   # relative, per cell per step
+# coolingh's neutral-fraction update (src/coolingh.f90:218-228) moves y by up
+# to 1.1e-12 (relative to rho) for a one-ulp change of exp() (measured over
+# 2e5 random states); the device libm and glibc differ by an ulp at times, so
+# the n_HI scalar under COOLINGH is held to ten times that.
+TOL_HI = 1e-11
 
 
 def _pair_run(cfg, nsteps, cap, admesh, tol=TOL, label=""):
@@ -38,9 +43,13 @@ def _pair_run(cfg, nsteps, cap, admesh, tol=TOL, label=""):
         assert abs(tg - t) <= TOL * t
         ug = sim.download()
         lv = m.leaves()
-        e, where = rel_err(ug[lv], m.u[lv], MASK)
+        e, where = rel_err(ug[lv][:, :4], m.u[lv][:, :4], MASK)
         worst = max(worst, e)
         assert e <= tol, f"{label} step {s}: rel err {e:.3e} at var/leaf/cell {where}"
+        if cfg.neq > 4:
+            e5, where5 = rel_err(ug[lv], m.u[lv], MASK)
+            lim = TOL_HI if cfg.cooling == "H" else tol
+            assert e5 <= lim, f"{label} step {s}: scalar rel err {e5:.3e} at {where5}"
         if admesh and cfg.nlevs > 1:
             assert O.admesh(pp, m) == 0
             sim.admesh()
@@ -53,3 +62,109 @@ def test_c1_explosion_steps():
     cfg = P.config_c1()
     w = _pair_run(cfg, 40, 64, admesh=False, label="c1")
     print("c1 worst rel err", w)
+
+
+def test_c2_sn_dmc_amr_steps():
+    """C2 physics (DMC cooling, 5 levels, outflow) on the full 8x8 root layout,
+    admesh every step on both sides; trees must stay identical."""
+    cfg = P.config_c2(nblocks=6000)
+    w = _pair_run(cfg, 25, 6000, admesh=True, label="c2")
+    print("c2 worst rel err", w)
+
+
+def test_c3_jet_coolingh_amr_steps():
+    """C3 physics (coolingh, jet source, reflective left/bottom) on a reduced
+    16x4 root layout, 4 levels, admesh every step."""
+    cfg = P.config_c3(nblocks=6000, nbrootx=16, nbrooty=4)
+    w = _pair_run(cfg, 25, 6000, admesh=True, label="c3")
+    print("c3 worst rel err", w)
+
+
+def _variant(**kw):
+    base = dict(name="variant", neq=5, nlevs=3, nbrootx=4, nbrooty=4, xmax=1.0, ymax=1.0, xphys=P.PC,
+                amol=0.6, cooling="ADIABATIC", bc=("REFLECTIVE", "OUTFLOW", "OUTFLOW", "REFLECTIVE"),
+                ic="explosion_code", ic_args=dict(rho=1.0, p=1e-2, E=1.0, r=0.1, xc=0.4, yc=0.45),
+                nblocks=3000)
+    base.update(kw)
+    return P.Config(**base)
+
+
+@pytest.mark.parametrize("kw", [
+    dict(solver="HLL"), dict(solver="HLL_HLLC"), dict(limiter=-1),
+    dict(limiter=0, ic_args=dict(rho=1.0, p=1.0, E=0.1, r=0.1, xc=0.4, yc=0.45)), dict(limiter=2),
+    dict(limiter=3), dict(limiter=4), dict(limiter=5), dict(limiter=6),
+    dict(cylindrical=True, limiter=2), dict(bc=("OUTFLOW", "REFLECTIVE", "REFLECTIVE", "OUTFLOW")),
+    dict(compat=0), dict(neq=6),
+], ids=lambda d: "-".join(f"{k}={v}" for k, v in d.items() if k != "ic_args"))
+def test_variants_amr(kw):
+    cfg = _variant(**kw)
+    _pair_run(cfg, 12, 3000, admesh=True, label=str(kw))
+
+
+def test_reference_interface_path():
+    """Drive the library the way the Fortran host does: the host keeps the tree
+    (here: the oracle), calls locatebounds itself and hands the list over
+    (wx_set_bounds), then wx_timestep / wx_tstep / wx_criteria, and replays
+    admesh's data ops through wx_refine / wx_coarse + wx_upload_tree."""
+    from harness import MASK
+    cfg = P.config_c2(nblocks=3000)
+    O = Oracle()
+    pp, m = oracle_mesh(O, cfg, 3000)
+    for _ in range(3):
+        O.admesh(pp, m)
+    sim = Simulation(cfg, nblocks=3000)
+    sim.upload(m.u, m.lp, m.leaf, m.icoord, m.nbmax, m.nbleafs)
+    t = 0.0
+    for s in range(10):
+        n = O.locatebounds(pp, m)
+        sim.set_bounds(m.bounds[:n])
+        dt_o = O.timestep(pp, m)
+        dt_g = sim.timestep()
+        assert abs(dt_g - dt_o) <= TOL * dt_o
+        t += dt_o
+        O.tstep(pp, m, t, dt_o)
+        sim.tstep(t, dt_o)
+        lv = m.leaves()
+        e, where = rel_err(sim.download()[lv], m.u[lv], MASK)
+        assert e <= TOL, (s, e, where)
+        # criteria flags from the device vs the oracle's criteria
+        up_g, dn_g = sim.criteria()
+        up_o = np.zeros(m.cap + 1, dtype=np.int32)
+        dn_o = np.zeros(m.cap + 1, dtype=np.int32)
+        for nb in lv:
+            O.criteria(pp, m, nb, up_o, dn_o)
+        assert np.array_equal(up_g[:m.cap], up_o[1:]) and np.array_equal(dn_g[:m.cap], dn_o[1:])
+        # admesh on the host tree; replay its data ops on the device
+        before = m.copy()
+        O.admesh(pp, m)
+        for nb in range(1, m.nbmax + 1):
+            s1 = m.lp[nb, 2]
+            if nb <= before.nbmax and before.leaf[nb] and not m.leaf[nb] and s1 and m.leaf[s1]:  # refined
+                s2 = m.lp[s1, 5]; s3 = m.lp[s1, 7]; s4 = m.lp[s3, 5]
+                sim.refine(nb, s1, s2, s3, s4)
+        for nb in range(1, m.nbmax + 1):
+            s1 = m.lp[nb, 2]
+            if (nb <= before.nbmax and not before.leaf[nb] and m.leaf[nb] and s1
+                    and before.leaf[s1] and not m.leaf[s1]):                 # coarsened into nb
+                s2 = m.lp[s1, 5]; s3 = m.lp[s1, 7]; s4 = m.lp[s3, 5]
+                sim.coarse(nb, s1, s2, s3, s4)
+        sim.upload_tree(m.lp, m.leaf, m.icoord, m.nbmax, m.nbleafs)
+        lv = m.leaves()
+        e, where = rel_err(sim.download()[lv], m.u[lv], MASK)
+        assert e <= TOL, ("after admesh", s, e, where)
+    sim.close()
+
+
+def test_fortran_layout_roundtrip():
+    """wx_upload / wx_download in the reference's block-fastest layout
+    u(nbmin:nbmaxtot, neq, -1:18, -1:18) (src/initmain.f90:76-78)."""
+    cfg = P.config_c1()
+    O = Oracle()
+    pp, m = oracle_mesh(O, cfg, 64)
+    sim = Simulation(cfg, nblocks=64)
+    sim.upload(m.u, m.lp, m.leaf, m.icoord, m.nbmax, m.nbleafs)
+    f = sim.download_fortran().reshape(20, 20, cfg.neq, m.nbmax)  # (j, i, k, nb) C-order view
+    for nb in (1, 7, m.nbmax):
+        for k in range(cfg.neq):
+            assert np.array_equal(f[:, :, k, nb - 1], m.u[nb, k])
+    sim.close()

@@ -27,8 +27,10 @@ EXPORTS = [
     "wx_create", "wx_destroy", "wx_upload", "wx_download", "wx_get_tree", "wx_set_bounds",
     "wx_locatebounds", "wx_timestep", "wx_tstep", "wx_criteria", "wx_refine", "wx_coarse",
     "wx_upload_tree", "wx_basegrid", "wx_admesh", "wx_step", "wx_set_time", "wx_info",
-    "wx_last_error", "wx_stream",
+    "wx_last_error", "wx_stream", "wx_profile",
 ]
+
+KERNELS = ["stageA", "stageB", "fillI_cool", "dtreduce", "blockstat", "meshops"]
 
 
 class WxError(RuntimeError):
@@ -66,6 +68,7 @@ def load_library(path: str = LIBPATH) -> ctypes.CDLL:
     L.wx_last_error.argtypes = [ctypes.c_char_p, ctypes.c_int32]
     L.wx_stream.argtypes = [P]
     L.wx_stream.restype = P
+    L.wx_profile.argtypes = [P, ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]
     _lib = L
     return L
 
@@ -221,6 +224,15 @@ class Simulation:
         ch = ctypes.c_int32()
         _check(self.L.wx_admesh(self.h, ctypes.byref(ch)), "wx_admesh")
         return ch.value
+
+    def profile(self, enable: int = -1):
+        """Per-kernel (ms, launches) since the last reset; enable 1/0 resets and
+        switches event recording on/off, -1 only reads."""
+        ms = np.zeros(8)
+        n = np.zeros(8, dtype=np.int64)
+        _check(self.L.wx_profile(self.h, enable, _d(ms), n.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 8),
+               "wx_profile")
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(KERNELS)}
 
     def step(self, nsteps: int = 1, admesh: bool = True):
         """nsteps iterations of src/main.f90:87-98; returns (time, dt, zone_updates)."""

@@ -94,6 +94,13 @@ struct wx_ctx {
   bool tables_valid = false;   // leaf list / binfo / ops match the tree and list
   bool bounds_external = false;
   bool stats_valid = false;
+  // per-kernel timing with HIP events on this context's stream
+  bool prof = false;
+  struct Mark { int kid; hipEvent_t a, b; };
+  std::vector<Mark> marks;
+  std::vector<hipEvent_t> evpool;
+  double prof_ms[8] = {0};
+  int64_t prof_n[8] = {0};
   // pinned staging
   DevScalars *hS = nullptr;
   int8_t *hFlag = nullptr;
@@ -104,6 +111,55 @@ struct wx_ctx {
   double *cur_u() { return U[cur].ptr; }
   double *other_u() { return U[cur ^ 1].ptr; }
 };
+
+namespace {
+
+// Kernel IDs for the profiling counters.
+enum { KID_STAGEA = 0, KID_STAGEB = 1, KID_FILLI = 2, KID_DTRED = 3, KID_STAT = 4, KID_MESH = 5 };
+
+hipEvent_t take_event(wx_ctx *c) {
+  if (!c->evpool.empty()) {
+    hipEvent_t e = c->evpool.back();
+    c->evpool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+struct ProfScope {
+  wx_ctx *c;
+  int kid;
+  hipEvent_t a = nullptr;
+  ProfScope(wx_ctx *c_, int kid_) : c(c_), kid(kid_) {
+    if (c->prof) { a = take_event(c); (void)hipEventRecord(a, c->stream); }
+  }
+  ~ProfScope() {
+    if (c->prof && a) {
+      hipEvent_t b = take_event(c);
+      (void)hipEventRecord(b, c->stream);
+      c->marks.push_back(wx_ctx::Mark{kid, a, b});
+    }
+  }
+};
+
+void prof_collect(wx_ctx *c) {
+  if (c->marks.empty()) return;
+  (void)hipStreamSynchronize(c->stream);
+  for (auto &m : c->marks) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, m.a, m.b) == hipSuccess) {
+      c->prof_ms[m.kid] += ms;
+      c->prof_n[m.kid] += 1;
+    }
+    c->evpool.push_back(m.a);
+    c->evpool.push_back(m.b);
+  }
+  c->marks.clear();
+}
+
+}  // namespace
 
 namespace {
 
@@ -293,6 +349,7 @@ int ensure_stats(wx_ctx *c) {
   int rc = ensure_tables(c);
   if (rc) return rc;
   if (c->stats_valid) return WX_OK;
+  ProfScope ps(c, KID_STAT);
   rc = launch_blockstat(c->dP.ptr, c->neq, c->dLeaves.ptr, (int)c->leaves.size(), c->cur_u(),
                         c->dStat.ptr, c->dFlag.ptr, c->stream);
   if (rc) return fail(rc, "blockstat launch failed");
@@ -312,11 +369,21 @@ int sync_scalars(wx_ctx *c) {
 
 int run_stages(wx_ctx *c) {
   const int nleaf = (int)c->leaves.size();
-  int rc = launch_stageA(c->dP.ptr, c->neq, c->dLeaves.ptr, nleaf, c->cur_u(), c->other_u(), c->dS.ptr, c->stream);
-  if (!rc) rc = launch_stageB(c->dP.ptr, c->neq, c->dLeaves.ptr, nleaf, c->dOpsII.ptr, c->dBinfo.ptr,
-                              c->cur_u(), c->other_u(), c->dS.ptr, c->stream);
-  if (!rc) rc = launch_fillI_cool(c->dP.ptr, c->neq, c->dLeaves.ptr, nleaf, c->dOpsI.ptr, c->dBinfo.ptr,
-                                  c->cur_u(), c->other_u(), c->dStat.ptr, c->dFlag.ptr, c->dS.ptr, c->stream);
+  int rc;
+  {
+    ProfScope ps(c, KID_STAGEA);
+    rc = launch_stageA(c->dP.ptr, c->neq, c->dLeaves.ptr, nleaf, c->cur_u(), c->other_u(), c->dS.ptr, c->stream);
+  }
+  if (!rc) {
+    ProfScope ps(c, KID_STAGEB);
+    rc = launch_stageB(c->dP.ptr, c->neq, c->dLeaves.ptr, nleaf, c->dOpsII.ptr, c->dBinfo.ptr,
+                       c->cur_u(), c->other_u(), c->dS.ptr, c->stream);
+  }
+  if (!rc) {
+    ProfScope ps(c, KID_FILLI);
+    rc = launch_fillI_cool(c->dP.ptr, c->neq, c->dLeaves.ptr, nleaf, c->dOpsI.ptr, c->dBinfo.ptr,
+                           c->cur_u(), c->other_u(), c->dStat.ptr, c->dFlag.ptr, c->dS.ptr, c->stream);
+  }
   if (rc) return fail(rc, "stage launch failed: %s", hipGetErrorString(hipGetLastError()));
   c->cur ^= 1;
   c->stats_valid = true;
@@ -434,7 +501,10 @@ int run_meshops(wx_ctx *c, const std::vector<MeshOp> &ops) {
   int rc = c->dMeshOps.ensure(ops.size());
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(c->dMeshOps.ptr, ops.data(), ops.size() * sizeof(MeshOp), hipMemcpyHostToDevice, c->stream));
-  rc = launch_meshops(c->dP.ptr, c->neq, c->dMeshOps.ptr, (int)ops.size(), c->cur_u(), c->stream);
+  {
+    ProfScope ps(c, KID_MESH);
+    rc = launch_meshops(c->dP.ptr, c->neq, c->dMeshOps.ptr, (int)ops.size(), c->cur_u(), c->stream);
+  }
   if (rc) return fail(rc, "meshops launch failed");
   HIPCHK(hipStreamSynchronize(c->stream));
   return WX_OK;
@@ -542,6 +612,8 @@ int wx_destroy(wx_ctx *c) {
   c->dP.release(); c->dS.release(); c->dLeaves.release(); c->dBinfo.release();
   c->dOpsII.release(); c->dOpsI.release(); c->dStat.release(); c->dFlag.release();
   c->dMeshOps.release(); c->dScratch.release();
+  prof_collect(c);
+  for (auto e : c->evpool) (void)hipEventDestroy(e);
   if (c->hS) (void)hipHostFree(c->hS);
   if (c->hFlag) (void)hipHostFree(c->hFlag);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -795,7 +867,10 @@ int wx_step(wx_ctx *c, int32_t nsteps, int32_t do_admesh_flag, double *time, dou
     int rc = ensure_stats(c);          // rebuilds tables after a mesh change
     if (rc) return rc;
     const int nleaf = (int)c->leaves.size();
-    rc = launch_dtreduce(c->dP.ptr, c->dStat.ptr, nleaf, c->dS.ptr, 1, c->stream);  // timestep; time += dt
+    {
+      ProfScope ps(c, KID_DTRED);
+      rc = launch_dtreduce(c->dP.ptr, c->dStat.ptr, nleaf, c->dS.ptr, 1, c->stream);  // timestep; time += dt
+    }
     if (rc) return fail(rc, "dtreduce launch failed");
     z += (double)nleaf * NX * NY;
     rc = run_stages(c);                // tstep
@@ -810,6 +885,20 @@ int wx_step(wx_ctx *c, int32_t nsteps, int32_t do_admesh_flag, double *time, dou
   if (time) *time = c->hS->time;
   if (dt) *dt = c->hS->dt;
   if (zones) *zones = z;
+  return WX_OK;
+}
+
+int wx_profile(wx_ctx *c, int32_t enable, double *ms, int64_t *counts, int32_t n) {
+  if (!c) return fail(WX_EARG, "null context");
+  prof_collect(c);
+  for (int k = 0; k < 8 && k < n; ++k) {
+    if (ms) ms[k] = c->prof_ms[k];
+    if (counts) counts[k] = c->prof_n[k];
+  }
+  if (enable >= 0) {
+    c->prof = enable != 0;
+    for (int k = 0; k < 8; ++k) { c->prof_ms[k] = 0; c->prof_n[k] = 0; }
+  }
   return WX_OK;
 }
 
