@@ -90,8 +90,12 @@ int wx_upload(wx_ctx *ctx, const double *u, int32_t layout, const int32_t *lp,
               const int32_t *leaf, const int32_t *icoord, int32_t ld,
               int32_t nbmax, int32_t nbleafs);
 /* Device -> host copy of u and of the primitives recomputed from it
- * (calcprim, src/calcprim.f90:5-21).  prim may be NULL. */
-int wx_download(wx_ctx *ctx, double *u, double *prim, int32_t layout);
+ * (calcprim, src/calcprim.f90:5-21).  prim may be NULL.  WX_LAYOUT_BLOCK
+ * writes tiles 0..nbmax; WX_LAYOUT_FORTRAN writes ld block rows (ld >= nbmax,
+ * the host array's leading dimension; ld <= 0 means nbmax).  The reference
+ * reads its arrays in place, so this is the transpose out of the device tiles
+ * that output.f90 (src/output.f90:39-67) needs before it writes. */
+int wx_download(wx_ctx *ctx, double *u, double *prim, int32_t layout, int32_t ld);
 /* Device -> host copy of the mesh tree kept by the context's own admesh. */
 int wx_get_tree(wx_ctx *ctx, int32_t *lp, int32_t *leaf, int32_t *icoord, int32_t ld,
                 int32_t *nbmax, int32_t *nbleafs);
@@ -116,6 +120,8 @@ int wx_tstep(wx_ctx *ctx, double time, double dt);
  * irefup/irefdown indexed by block ID - 1, length nblocks.  Synchronising. */
 int wx_criteria(wx_ctx *ctx, int32_t *irefup, int32_t *irefdown);
 
+/* wx_upload_tree keeps the compiled tables and block statistics when the tree
+ * is unchanged, so a host may call it every step. */
 /* Data parts of src/refine.f90:30-42 (injection over the padded block) and
  * src/coarse.f90:182-366 (2x2 restriction incl. ghosts).  The caller keeps
  * its own tree (updatelpup/updatelpdown) and calls wx_upload_tree after. */

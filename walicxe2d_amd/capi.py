@@ -50,7 +50,7 @@ def load_library(path: str = LIBPATH) -> ctypes.CDLL:
     L.wx_create.argtypes = [ctypes.POINTER(WxParams), ctypes.POINTER(P)]
     L.wx_destroy.argtypes = [P]
     L.wx_upload.argtypes = [P, _dp, ctypes.c_int32, _ip, _ip, _ip, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
-    L.wx_download.argtypes = [P, _dp, _dp, ctypes.c_int32]
+    L.wx_download.argtypes = [P, _dp, _dp, ctypes.c_int32, ctypes.c_int32]
     L.wx_get_tree.argtypes = [P, _ip, _ip, _ip, ctypes.c_int32, _ip, _ip]
     L.wx_set_bounds.argtypes = [P, _ip, ctypes.c_int32, ctypes.c_int32]
     L.wx_locatebounds.argtypes = [P, _ip]
@@ -161,13 +161,14 @@ class Simulation:
         nbmax = self.info()[0]
         u = np.zeros((nbmax + 1, self.neq, NP, NP))
         pr = np.zeros_like(u) if prim else None
-        _check(self.L.wx_download(self.h, _d(u), _d(pr) if prim else None, WX_LAYOUT_BLOCK), "wx_download")
+        _check(self.L.wx_download(self.h, _d(u), _d(pr) if prim else None, WX_LAYOUT_BLOCK, 0), "wx_download")
         return (u, pr) if prim else u
 
-    def download_fortran(self):
+    def download_fortran(self, ld: int = 0):
         nbmax = self.info()[0]
-        u = np.zeros(nbmax * self.neq * NCELL)
-        _check(self.L.wx_download(self.h, _d(u), None, WX_LAYOUT_FORTRAN), "wx_download")
+        ld = ld or nbmax
+        u = np.zeros(ld * self.neq * NCELL)
+        _check(self.L.wx_download(self.h, _d(u), None, WX_LAYOUT_FORTRAN, ld), "wx_download")
         return u
 
     def tree(self):

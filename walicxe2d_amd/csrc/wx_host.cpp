@@ -624,9 +624,20 @@ int wx_destroy(wx_ctx *c) {
 void *wx_stream(wx_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
 static int set_tree(wx_ctx *c, const int32_t *lp, const int32_t *leaf, const int32_t *icoord, int32_t ld,
-                    int32_t nbmax, int32_t nbleafs) {
+                    int32_t nbmax, int32_t nbleafs, bool *changed = nullptr) {
   if (!lp || !leaf || !icoord) return fail(WX_EARG, "null tree array");
   if (nbmax < 0 || nbmax > c->cap || ld < nbmax) return fail(WX_EARG, "nbmax %d / ld %d vs capacity %d", nbmax, ld, c->cap);
+  bool same = (nbmax == c->nbmax);
+  for (int nb = 1; same && nb <= nbmax; ++nb) {
+    for (int k = 1; k <= 8 && same; ++k) same = c->LP(nb, k) == lp[(size_t)(k - 1) * ld + (nb - 1)];
+    same = same && (c->leaf[nb] == (leaf[nb - 1] != 0)) && c->icoord[(size_t)nb * 2] == icoord[nb - 1] &&
+           c->icoord[(size_t)nb * 2 + 1] == icoord[(size_t)ld + (nb - 1)];
+  }
+  if (changed) *changed = !same;
+  if (same) {
+    c->nbleafs = nbleafs;
+    return WX_OK;
+  }
   std::fill(c->lp.begin(), c->lp.end(), 0);
   std::fill(c->leaf.begin(), c->leaf.end(), 0);
   std::fill(c->icoord.begin(), c->icoord.end(), 0);
@@ -649,6 +660,7 @@ int wx_upload(wx_ctx *c, const double *u, int32_t layout, const int32_t *lp, con
   if (!c || !u) return fail(WX_EARG, "null argument");
   int rc = set_tree(c, lp, leaf, icoord, ld, nbmax, nbleafs);
   if (rc) return rc;
+  c->stats_valid = false;
   const size_t tiles = (size_t)(c->cap + 1) * c->neq * NC;
   if (layout == WX_LAYOUT_BLOCK) {
     const size_t n = (size_t)(nbmax + 1) * c->neq * NC;
@@ -691,7 +703,7 @@ int wx_get_tree(wx_ctx *c, int32_t *lp, int32_t *leaf, int32_t *icoord, int32_t 
   return WX_OK;
 }
 
-int wx_download(wx_ctx *c, double *u, double *prim, int32_t layout) {
+int wx_download(wx_ctx *c, double *u, double *prim, int32_t layout, int32_t ld_in) {
   if (!c) return fail(WX_EARG, "null context");
   int rc;
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -710,7 +722,8 @@ int wx_download(wx_ctx *c, double *u, double *prim, int32_t layout) {
       HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
     } else if (layout == WX_LAYOUT_FORTRAN) {
-      const int ld = c->nbmax;
+      const int ld = ld_in > 0 ? ld_in : c->nbmax;
+      if (ld < c->nbmax || ld > c->cap) return fail(WX_EARG, "download: ld %d outside [nbmax %d, capacity %d]", ld, c->nbmax, c->cap);
       const size_t m = (size_t)ld * c->neq * NC;
       DevBuf<double> tmp;
       if ((rc = tmp.ensure(m))) return rc;
