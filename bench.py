@@ -10,7 +10,9 @@ warm-up steps, bracketed by device synchronisation.
 
 Prints one JSON line (see the keys below).  Options:
   --gpus N          only N = 1 is supported (one GPU holds every leaf block)
-  --steps K --warmup W
+  --steps K --warmup W   default K=12000, W=50: the cooling jet from t=0 to
+                         about 300 yr (the configured end time), i.e. the
+                         whole run with its growing AMR mesh
   --workload NAME   c3 (default: the variable cooling jet), c2, c1
   --no-cpu-baseline skip the CPU oracle timing
 """
@@ -73,8 +75,8 @@ def cpu_baseline(cfg, seconds: float = 12.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
-    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=12000)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="c3")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()

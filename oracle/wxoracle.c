@@ -1026,11 +1026,19 @@ int wxo_admesh(const wxo_params *p, wxo_mesh *m) {
   int n = m->cap + 1;
   int32_t *up = (int32_t *)calloc((size_t)n, sizeof(int32_t));
   int32_t *dn = (int32_t *)calloc((size_t)n, sizeof(int32_t));
-  int err = 0;
-  int nbmaxp = m->nbmax, nbmaxp1 = m->nbmax;
   /* markref */
   for (int nb = 1; nb <= m->nbmax; ++nb)
     if (m->leaf[nb] && LP(m, nb, 1) >= 1) wxo_criteria(p, m, nb, up, dn);
+  int err = wxo_admesh_flags(p, m, up, dn);
+  free(up);
+  free(dn);
+  return err;
+}
+
+/* admesh after markref, from given flags (irefup/irefdown by block ID). */
+int wxo_admesh_flags(const wxo_params *p, wxo_mesh *m, int32_t *up, int32_t *dn) {
+  int err = 0;
+  int nbmaxp = m->nbmax, nbmaxp1 = m->nbmax;
   /* critneighup */
   for (int nl = p->nlevs - 1; nl >= 1; --nl)
     for (int nb = 1; nb <= m->nbmax; ++nb)
@@ -1075,8 +1083,6 @@ int wxo_admesh(const wxo_params *p, wxo_mesh *m) {
         }
       }
   }
-  free(up);
-  free(dn);
   return err;
 }
 

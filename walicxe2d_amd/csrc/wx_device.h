@@ -1,5 +1,6 @@
 // wx_device.h -- structures shared by the host runtime and the gfx950 kernels.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "../../include/wx_capi.h"
 
@@ -35,7 +36,8 @@ struct LeafRec {
   int32_t level;
   int32_t icx, icy;
   int32_t flags;   // bit0: block imposed by the jet source in this step's fills
-  int32_t pad[3];
+  int32_t lidx;    // position in the leaf list
+  int32_t pad[2];
 };
 
 // Per-ID record for source lookups (level and lower-left cell index).
@@ -71,8 +73,8 @@ int launch_fillI_cool(const wx_params *dP, int neq, const LeafRec *leaves, int n
                       double *stat, int8_t *flag, DevScalars *S, void *stream);
 int launch_blockstat(const wx_params *dP, int neq, const LeafRec *leaves, int nleaf,
                      const double *U, double *stat, int8_t *flag, void *stream);
-int launch_dtreduce(const wx_params *dP, const double *stat, int nleaf, DevScalars *S,
-                    int advance_time, void *stream);
+int launch_dtreduce(const wx_params *dP, const LeafRec *leaves, const double *stat, int nleaf,
+                    DevScalars *S, int advance_time, void *stream);
 int launch_meshops(const wx_params *dP, int neq, const MeshOp *ops, int nops, double *U,
                    void *stream);
 int launch_transpose(int neq, int cap, const double *src, double *dst, int to_fortran,

@@ -93,7 +93,19 @@ struct wx_ctx {
   std::vector<int> lidx;
   bool tables_valid = false;   // leaf list / binfo / ops match the tree and list
   bool bounds_external = false;
-  bool stats_valid = false;
+  // block statistics (CFL partials + refinement flag) are kept per block ID;
+  // after a mesh change only the new leaves are recomputed
+  bool stats_all_stale = true;
+  std::vector<int> new_ids;
+  std::vector<LeafRec> dirty;
+  DevBuf<LeafRec> dDirty;
+  // pinned host staging for uploads: two buffers, each guarded by an event,
+  // so a table upload never waits for the device
+  struct Stage { char *buf = nullptr; size_t cap = 0, used = 0; hipEvent_t ev = nullptr; bool pending = false; };
+  Stage stage[2];
+  int stage_cur = 0;
+  // admesh scratch (flags per block ID)
+  std::vector<uint8_t> up, dn;
   // per-kernel timing with HIP events on this context's stream
   bool prof = false;
   struct Mark { int kid; hipEvent_t a, b; };
@@ -159,6 +171,47 @@ void prof_collect(wx_ctx *c) {
   c->marks.clear();
 }
 
+// Opens the next staging buffer for `total` bytes (waits only if that buffer's
+// previous copies have not finished).
+int stage_begin(wx_ctx *c, size_t total) {
+  c->stage_cur ^= 1;
+  wx_ctx::Stage &st = c->stage[c->stage_cur];
+  if (st.pending) {
+    (void)hipEventSynchronize(st.ev);
+    st.pending = false;
+  }
+  if (!st.ev && hipEventCreateWithFlags(&st.ev, hipEventDisableTiming) != hipSuccess)
+    return fail(WX_EHIP, "hipEventCreate failed");
+  if (st.cap < total) {
+    if (st.buf) (void)hipHostFree(st.buf);
+    st.buf = nullptr;
+    st.cap = 0;
+    size_t want = total + total / 2 + 4096;
+    if (hipHostMalloc((void **)&st.buf, want) != hipSuccess) return fail(WX_ENOMEM, "pinned staging %zu bytes", want);
+    st.cap = want;
+  }
+  st.used = 0;
+  return WX_OK;
+}
+
+int stage_copy(wx_ctx *c, void *dst, const void *src, size_t bytes) {
+  if (!bytes) return WX_OK;
+  wx_ctx::Stage &st = c->stage[c->stage_cur];
+  char *p = st.buf + st.used;
+  memcpy(p, src, bytes);
+  st.used += (bytes + 255) & ~(size_t)255;
+  HIPCHK(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, c->stream));
+  return WX_OK;
+}
+
+void stage_end(wx_ctx *c) {
+  wx_ctx::Stage &st = c->stage[c->stage_cur];
+  (void)hipEventRecord(st.ev, c->stream);
+  st.pending = true;
+}
+
+size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
 }  // namespace
 
 namespace {
@@ -166,8 +219,7 @@ namespace {
 // ---------------------------------------------------------------------
 // src/locatebounds.f90:29-540, MPIP forms, one rank.
 void locatebounds(wx_ctx *c) {
-  const int cap = c->cap;
-  std::vector<uint8_t> bnd((size_t)(cap + 1) * 4, 0);
+  std::vector<uint8_t> bnd((size_t)(c->nbmax + 1) * 4, 0);
   c->bounds.clear();
   static const int opp[4] = {1, 0, 3, 2};
   auto add = [&](int a, int b, int code) {
@@ -215,7 +267,7 @@ struct Seg {
 };
 
 int compile_tables(wx_ctx *c) {
-  const int cap = c->cap;
+  const int cap = c->nbmax;   // IDs above nbmax hold no block
   c->leaves.clear();
   c->lidx.assign((size_t)cap + 1, -1);
   c->binfo.assign((size_t)cap + 1, BlockRec{0, 0, 0, 0});
@@ -228,6 +280,7 @@ int compile_tables(wx_ctx *c) {
     r.level = c->LP(nb, 1);
     r.icx = c->icoord[(size_t)nb * 2];
     r.icy = c->icoord[(size_t)nb * 2 + 1];
+    r.lidx = (int)c->leaves.size();
     if (r.level < 1 || r.level > c->p.nlevs) return fail(WX_EARG, "block %d has level %d", nb, r.level);
     c->leaves.push_back(r);
   }
@@ -321,16 +374,13 @@ int upload_tables(wx_ctx *c) {
   if ((rc = c->dBinfo.ensure(c->binfo.size()))) return rc;
   if ((rc = c->dOpsII.ensure(nleaf * 8))) return rc;
   if ((rc = c->dOpsI.ensure(nleaf * 8))) return rc;
-  if ((rc = c->dStat.ensure(nleaf * 2))) return rc;
-  if ((rc = c->dFlag.ensure(nleaf))) return rc;
-  if (nleaf) {
-    HIPCHK(hipMemcpyAsync(c->dLeaves.ptr, c->leaves.data(), nleaf * sizeof(LeafRec), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->dOpsII.ptr, c->opsII.data(), nleaf * 8 * sizeof(int2), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->dOpsI.ptr, c->opsI.data(), nleaf * 8 * sizeof(int2), hipMemcpyHostToDevice, c->stream));
-  }
-  HIPCHK(hipMemcpyAsync(c->dBinfo.ptr, c->binfo.data(), c->binfo.size() * sizeof(BlockRec), hipMemcpyHostToDevice, c->stream));
-  // the vectors above must stay alive until the copies ran
-  HIPCHK(hipStreamSynchronize(c->stream));
+  const size_t bl = nleaf * sizeof(LeafRec), bo = nleaf * 8 * sizeof(int2), bb = c->binfo.size() * sizeof(BlockRec);
+  if ((rc = stage_begin(c, al256(bl) + 2 * al256(bo) + al256(bb)))) return rc;
+  if ((rc = stage_copy(c, c->dLeaves.ptr, c->leaves.data(), bl))) return rc;
+  if ((rc = stage_copy(c, c->dOpsII.ptr, c->opsII.data(), bo))) return rc;
+  if ((rc = stage_copy(c, c->dOpsI.ptr, c->opsI.data(), bo))) return rc;
+  if ((rc = stage_copy(c, c->dBinfo.ptr, c->binfo.data(), bb))) return rc;
+  stage_end(c);
   return WX_OK;
 }
 
@@ -348,12 +398,29 @@ int ensure_tables(wx_ctx *c) {
 int ensure_stats(wx_ctx *c) {
   int rc = ensure_tables(c);
   if (rc) return rc;
-  if (c->stats_valid) return WX_OK;
+  if (c->stats_all_stale) {
+    ProfScope ps(c, KID_STAT);
+    rc = launch_blockstat(c->dP.ptr, c->neq, c->dLeaves.ptr, (int)c->leaves.size(), c->cur_u(),
+                          c->dStat.ptr, c->dFlag.ptr, c->stream);
+    if (rc) return fail(rc, "blockstat launch failed");
+    c->stats_all_stale = false;
+    c->new_ids.clear();
+    return WX_OK;
+  }
+  if (c->new_ids.empty()) return WX_OK;
+  c->dirty.clear();
+  for (int id : c->new_ids)
+    if (id >= 1 && id <= c->nbmax && c->lidx[id] >= 0) c->dirty.push_back(c->leaves[c->lidx[id]]);
+  c->new_ids.clear();
+  if (c->dirty.empty()) return WX_OK;
+  if ((rc = c->dDirty.ensure(c->dirty.size()))) return rc;
+  if ((rc = stage_begin(c, al256(c->dirty.size() * sizeof(LeafRec))))) return rc;
+  if ((rc = stage_copy(c, c->dDirty.ptr, c->dirty.data(), c->dirty.size() * sizeof(LeafRec)))) return rc;
+  stage_end(c);
   ProfScope ps(c, KID_STAT);
-  rc = launch_blockstat(c->dP.ptr, c->neq, c->dLeaves.ptr, (int)c->leaves.size(), c->cur_u(),
-                        c->dStat.ptr, c->dFlag.ptr, c->stream);
+  rc = launch_blockstat(c->dP.ptr, c->neq, c->dDirty.ptr, (int)c->dirty.size(), c->cur_u(), c->dStat.ptr,
+                        c->dFlag.ptr, c->stream);
   if (rc) return fail(rc, "blockstat launch failed");
-  c->stats_valid = true;
   return WX_OK;
 }
 
@@ -386,7 +453,8 @@ int run_stages(wx_ctx *c) {
   }
   if (rc) return fail(rc, "stage launch failed: %s", hipGetErrorString(hipGetLastError()));
   c->cur ^= 1;
-  c->stats_valid = true;
+  c->stats_all_stale = false;   // the epilogue of the last kernel wrote every leaf's statistics
+  c->new_ids.clear();
   return WX_OK;
 }
 
@@ -500,26 +568,28 @@ int run_meshops(wx_ctx *c, const std::vector<MeshOp> &ops) {
   if (ops.empty()) return WX_OK;
   int rc = c->dMeshOps.ensure(ops.size());
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(c->dMeshOps.ptr, ops.data(), ops.size() * sizeof(MeshOp), hipMemcpyHostToDevice, c->stream));
+  if ((rc = stage_begin(c, al256(ops.size() * sizeof(MeshOp))))) return rc;
+  if ((rc = stage_copy(c, c->dMeshOps.ptr, ops.data(), ops.size() * sizeof(MeshOp)))) return rc;
+  stage_end(c);
   {
     ProfScope ps(c, KID_MESH);
     rc = launch_meshops(c->dP.ptr, c->neq, c->dMeshOps.ptr, (int)ops.size(), c->cur_u(), c->stream);
   }
   if (rc) return fail(rc, "meshops launch failed");
-  HIPCHK(hipStreamSynchronize(c->stream));
   return WX_OK;
 }
 
 int fetch_flags(wx_ctx *c) {
-  const size_t nleaf = c->leaves.size();
-  if (c->hFlagN < nleaf) {
+  const size_t n = (size_t)c->nbmax + 1;
+  if (c->hFlagN < n) {
     if (c->hFlag) (void)hipHostFree(c->hFlag);
     c->hFlag = nullptr;
     c->hFlagN = 0;
-    HIPCHK(hipHostMalloc((void **)&c->hFlag, std::max<size_t>(nleaf, 1) * 2));
-    c->hFlagN = std::max<size_t>(nleaf, 1) * 2;
+    const size_t want = (size_t)c->cap + 1;
+    HIPCHK(hipHostMalloc((void **)&c->hFlag, want));
+    c->hFlagN = want;
   }
-  if (nleaf) HIPCHK(hipMemcpyAsync(c->hFlag, c->dFlag.ptr, nleaf, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->hFlag, c->dFlag.ptr, n, hipMemcpyDeviceToHost, c->stream));
   return sync_scalars(c);
 }
 
@@ -528,21 +598,29 @@ int do_admesh(wx_ctx *c, int *changed) {
   if (rc) return rc;
   rc = fetch_flags(c);
   if (rc) return rc;
-  std::vector<uint8_t> up((size_t)c->cap + 1, 0), dn((size_t)c->cap + 1, 0);
-  for (size_t l = 0; l < c->leaves.size(); ++l) {   // markref, src/markref.f90:37-44
-    const int nb = c->leaves[l].id;
-    up[nb] = c->hFlag[l] == 1;
-    dn[nb] = c->hFlag[l] == 2;
+  std::vector<uint8_t> &up = c->up, &dn = c->dn;
+  if (up.size() != (size_t)c->cap + 1) {
+    up.assign((size_t)c->cap + 1, 0);
+    dn.assign((size_t)c->cap + 1, 0);
+  }
+  for (const LeafRec &r : c->leaves) {   // markref, src/markref.f90:37-44
+    up[r.id] = c->hFlag[r.id] == 1;
+    dn[r.id] = c->hFlag[r.id] == 2;
   }
   std::vector<MeshOp> ops;
   rc = admesh_tree(c, up, dn, ops);
+  std::fill(up.begin(), up.begin() + c->nbmax + 1, 0);
+  std::fill(dn.begin(), dn.begin() + c->nbmax + 1, 0);
   if (rc) return rc;
   if (changed) *changed = (int)ops.size();
   if (!ops.empty()) {
     rc = run_meshops(c, ops);
     if (rc) return rc;
+    for (const MeshOp &o : ops) {
+      if (o.kind == 0) { c->new_ids.push_back(o.s1); c->new_ids.push_back(o.s2); c->new_ids.push_back(o.s3); c->new_ids.push_back(o.s4); }
+      else c->new_ids.push_back(o.a);
+    }
     c->tables_valid = false;
-    c->stats_valid = false;
     c->bounds_external = false;
   }
   return WX_OK;
@@ -591,6 +669,8 @@ int wx_create(const wx_params *p, wx_ctx **out) {
   for (int k = 0; k < 2 && !rc; ++k) rc = c->U[k].ensure(tiles);
   if (!rc) rc = c->dP.ensure(1);
   if (!rc) rc = c->dS.ensure(1);
+  if (!rc) rc = c->dStat.ensure((size_t)(c->cap + 1) * 2);
+  if (!rc) rc = c->dFlag.ensure((size_t)c->cap + 1);
   if (rc) { wx_destroy(c); return rc; }
   for (int k = 0; k < 2; ++k) (void)hipMemsetAsync(c->U[k].ptr, 0, tiles * sizeof(double), c->stream);
   (void)hipMemcpyAsync(c->dP.ptr, p, sizeof(wx_params), hipMemcpyHostToDevice, c->stream);
@@ -611,7 +691,11 @@ int wx_destroy(wx_ctx *c) {
   for (auto &b : c->U) b.release();
   c->dP.release(); c->dS.release(); c->dLeaves.release(); c->dBinfo.release();
   c->dOpsII.release(); c->dOpsI.release(); c->dStat.release(); c->dFlag.release();
-  c->dMeshOps.release(); c->dScratch.release();
+  c->dMeshOps.release(); c->dScratch.release(); c->dDirty.release();
+  for (auto &st : c->stage) {
+    if (st.ev) (void)hipEventDestroy(st.ev);
+    if (st.buf) (void)hipHostFree(st.buf);
+  }
   prof_collect(c);
   for (auto e : c->evpool) (void)hipEventDestroy(e);
   if (c->hS) (void)hipHostFree(c->hS);
@@ -650,7 +734,7 @@ static int set_tree(wx_ctx *c, const int32_t *lp, const int32_t *leaf, const int
   c->nbmax = nbmax;
   c->nbleafs = nbleafs;
   c->tables_valid = false;
-  c->stats_valid = false;
+  c->stats_all_stale = true;
   c->bounds_external = false;
   return WX_OK;
 }
@@ -660,7 +744,7 @@ int wx_upload(wx_ctx *c, const double *u, int32_t layout, const int32_t *lp, con
   if (!c || !u) return fail(WX_EARG, "null argument");
   int rc = set_tree(c, lp, leaf, icoord, ld, nbmax, nbleafs);
   if (rc) return rc;
-  c->stats_valid = false;
+  c->stats_all_stale = true;
   const size_t tiles = (size_t)(c->cap + 1) * c->neq * NC;
   if (layout == WX_LAYOUT_BLOCK) {
     const size_t n = (size_t)(nbmax + 1) * c->neq * NC;
@@ -770,7 +854,7 @@ int wx_timestep(wx_ctx *c, double *dt) {
   if (!c || !dt) return fail(WX_EARG, "null argument");
   int rc = ensure_stats(c);
   if (rc) return rc;
-  rc = launch_dtreduce(c->dP.ptr, c->dStat.ptr, (int)c->leaves.size(), c->dS.ptr, 0, c->stream);
+  rc = launch_dtreduce(c->dP.ptr, c->dLeaves.ptr, c->dStat.ptr, (int)c->leaves.size(), c->dS.ptr, 0, c->stream);
   if (rc) return fail(rc, "dtreduce launch failed");
   rc = sync_scalars(c);
   if (rc) return rc;
@@ -797,10 +881,9 @@ int wx_criteria(wx_ctx *c, int32_t *irefup, int32_t *irefdown) {
   rc = fetch_flags(c);
   if (rc) return rc;
   for (int nb = 1; nb <= c->cap; ++nb) { irefup[nb - 1] = 0; irefdown[nb - 1] = 0; }
-  for (size_t l = 0; l < c->leaves.size(); ++l) {
-    const int nb = c->leaves[l].id;
-    irefup[nb - 1] = c->hFlag[l] == 1;
-    irefdown[nb - 1] = c->hFlag[l] == 2;
+  for (const LeafRec &r : c->leaves) {
+    irefup[r.id - 1] = c->hFlag[r.id] == 1;
+    irefdown[r.id - 1] = c->hFlag[r.id] == 2;
   }
   return WX_OK;
 }
@@ -810,7 +893,7 @@ int wx_refine(wx_ctx *c, int32_t nb, int32_t s1, int32_t s2, int32_t s3, int32_t
   const int ids[5] = {nb, s1, s2, s3, s4};
   for (int k = 0; k < 5; ++k) if (ids[k] < 1 || ids[k] > c->cap) return fail(WX_EARG, "refine: block %d out of range", ids[k]);
   std::vector<MeshOp> ops{MeshOp{0, nb, s1, s2, s3, s4, {0, 0}}};
-  c->stats_valid = false;
+  c->stats_all_stale = true;
   c->tables_valid = false;
   return run_meshops(c, ops);
 }
@@ -820,7 +903,7 @@ int wx_coarse(wx_ctx *c, int32_t dad, int32_t s1, int32_t s2, int32_t s3, int32_
   const int ids[5] = {dad, s1, s2, s3, s4};
   for (int k = 0; k < 5; ++k) if (ids[k] < 1 || ids[k] > c->cap) return fail(WX_EARG, "coarse: block %d out of range", ids[k]);
   std::vector<MeshOp> ops{MeshOp{1, dad, s1, s2, s3, s4, {0, 0}}};
-  c->stats_valid = false;
+  c->stats_all_stale = true;
   c->tables_valid = false;
   return run_meshops(c, ops);
 }
@@ -851,7 +934,7 @@ int wx_basegrid(wx_ctx *c, const double *uroots, int32_t nroots) {
   HIPCHK(hipMemcpyAsync(c->cur_u(), uroots, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->tables_valid = false;
-  c->stats_valid = false;
+  c->stats_all_stale = true;
   c->bounds_external = false;
   return WX_OK;
 }
@@ -882,7 +965,7 @@ int wx_step(wx_ctx *c, int32_t nsteps, int32_t do_admesh_flag, double *time, dou
     const int nleaf = (int)c->leaves.size();
     {
       ProfScope ps(c, KID_DTRED);
-      rc = launch_dtreduce(c->dP.ptr, c->dStat.ptr, nleaf, c->dS.ptr, 1, c->stream);  // timestep; time += dt
+      rc = launch_dtreduce(c->dP.ptr, c->dLeaves.ptr, c->dStat.ptr, nleaf, c->dS.ptr, 1, c->stream);  // timestep; time += dt
     }
     if (rc) return fail(rc, "dtreduce launch failed");
     z += (double)nleaf * NX * NY;

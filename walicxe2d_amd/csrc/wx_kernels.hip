@@ -316,11 +316,11 @@ __global__ __launch_bounds__(256) void k_stageB(const wx_params *__restrict__ p,
 // ---------------------------------------------------------------------
 // Block statistics: CFL partials (src/timestep.f90:29-41) and the
 // refinement flag (src/criteria.f90:16-50) from a tile of primitives.
-// stat[2l] = min dx/(|u|+cs), min dy/(|v|+cs) ; stat[2l+1] = DMC t_cool min.
-// flag: 1 refine, 2 coarsen, 0 neither.
+// Indexed by block ID: stat[2id] = min dx/(|u|+cs), dy/(|v|+cs);
+// stat[2id+1] = DMC t_cool min; flag[id]: 1 refine, 2 coarsen, 0 neither.
 template <int NEQ>
 __device__ void block_stats(const wx_params *__restrict__ p, const double *prim, const LeafRec &L,
-                            double *stat, int8_t *flag, int l) {
+                            double *stat, int8_t *flag) {
   __shared__ double red[3][4];
   const int tid = threadIdx.x;
   const int i = tid % 16 + 1, j = tid / 16 + 1;
@@ -361,12 +361,12 @@ __device__ void block_stats(const wx_params *__restrict__ p, const double *prim,
       dtc = dmin(dtc, red[1][k]);
       grad = dmax(grad, red[2][k]);
     }
-    stat[2 * l] = dtp;
-    stat[2 * l + 1] = dtc;
+    stat[2 * L.id] = dtp;
+    stat[2 * L.id + 1] = dtc;
     int8_t fl = 0;
     if (grad >= p->tol) fl = (lev < p->nlevs) ? 1 : 0;
     else if (grad <= p->tol2) fl = 2;
-    flag[l] = fl;
+    flag[L.id] = fl;
   }
 }
 
@@ -448,7 +448,7 @@ __global__ __launch_bounds__(256) void k_fillI_cool(const wx_params *__restrict_
     for (int k = 0; k < NEQ; ++k) tile[k * NC + IDX(i, j)] = q[k];
   }
   __syncthreads();
-  block_stats<NEQ>(p, tile, L, stat, flag, l);
+  block_stats<NEQ>(p, tile, L, stat, flag);
 }
 
 // Statistics only (after upload or a mesh change).
@@ -472,19 +472,21 @@ __global__ __launch_bounds__(256) void k_blockstat(const wx_params *__restrict__
     for (int k = 0; k < NEQ; ++k) tile[k * NC + IDX(i, j)] = q[k];
   }
   __syncthreads();
-  block_stats<NEQ>(p, tile, L, stat, flag, l);
+  block_stats<NEQ>(p, tile, L, stat, flag);
 }
 
 // dt = min(cfl*dtp, 0.4*dtcool*Psc/tsc) (src/timestep.f90:50); optionally
 // time += dt (src/main.f90:88).  One workgroup.
 __global__ __launch_bounds__(1024) void k_dtreduce(const wx_params *__restrict__ p,
+                                                   const LeafRec *__restrict__ leaves,
                                                    const double *__restrict__ stat, int nleaf,
                                                    DevScalars *__restrict__ S, int advance) {
   __shared__ double r0[16], r1[16];
   double a = 1.e30, c = 1e30;
   for (int l = threadIdx.x; l < nleaf; l += blockDim.x) {
-    a = dmin(a, stat[2 * l]);
-    c = dmin(c, stat[2 * l + 1]);
+    const int id = leaves[l].id;
+    a = dmin(a, stat[2 * id]);
+    c = dmin(c, stat[2 * id + 1]);
   }
   for (int o = 32; o > 0; o >>= 1) {
     a = dmin(a, __shfl_xor(a, o));
@@ -643,9 +645,9 @@ int launch_blockstat(const wx_params *dP, int neq, const LeafRec *leaves, int nl
   return hip_status();
 }
 
-int launch_dtreduce(const wx_params *dP, const double *stat, int nleaf, DevScalars *S, int advance,
-                    void *stream) {
-  hipLaunchKernelGGL(k_dtreduce, dim3(1), dim3(1024), 0, (hipStream_t)stream, dP, stat, nleaf, S, advance);
+int launch_dtreduce(const wx_params *dP, const LeafRec *leaves, const double *stat, int nleaf, DevScalars *S,
+                    int advance, void *stream) {
+  hipLaunchKernelGGL(k_dtreduce, dim3(1), dim3(1024), 0, (hipStream_t)stream, dP, leaves, stat, nleaf, S, advance);
   return hip_status();
 }
 
