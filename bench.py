@@ -111,11 +111,16 @@ def main():
     ms, launches = prof[dom]
     alg_bytes = ALG_PASSES[dom] * neq * 8 * zones         # over all launches of the kernel
     achieved = alg_bytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    # measured HBM bytes per launch: the copy-calibrated FETCH_SIZE + WRITE_SIZE
+    # per zone-update of this kernel on this workload (rocprofv3 --pmc runs
+    # summarised in profiles/), times the zone-updates of one launch here
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
-            traffic = json.load(fh).get(f"{dom}:{args.workload}")
+            bpz = json.load(fh).get("per_zone_update", {}).get(f"{dom}:{args.workload}")
+        if bpz is not None:
+            traffic = round(bpz * zones / max(launches, 1))
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": dom, "kernel_ms_per_launch": round(ms / max(launches, 1), 5),

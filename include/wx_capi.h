@@ -148,7 +148,9 @@ int wx_set_time(wx_ctx *ctx, double time);
  * every launch (no counterpart in the reference, which has no timers).
  * Returns the accumulated milliseconds / launch counts of kernels
  * 0 stage A, 1 stage B, 2 boundaryI+cooling, 3 dt reduction, 4 block
- * statistics, 5 refine/coarse ops (n entries); then, if enable >= 0, resets
+ * statistics, 5 refine/coarse ops; 6 host wall time waiting for the flags and
+ * running admesh, 7 host wall time compiling and uploading the ghost lists
+ * (n entries); then, if enable >= 0, resets
  * the counters and switches recording on (1) or off (0). */
 int wx_profile(wx_ctx *ctx, int32_t enable, double *ms, int64_t *counts, int32_t n);
 

@@ -30,7 +30,7 @@ EXPORTS = [
     "wx_last_error", "wx_stream", "wx_profile",
 ]
 
-KERNELS = ["stageA", "stageB", "fillI_cool", "dtreduce", "blockstat", "meshops"]
+KERNELS = ["stageA", "stageB", "fillI_cool", "dtreduce", "blockstat", "meshops", "host_admesh", "host_tables"]
 
 
 class WxError(RuntimeError):

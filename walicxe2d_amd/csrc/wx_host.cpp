@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cstdarg>
 #include <cstdio>
@@ -128,6 +129,17 @@ namespace {
 
 // Kernel IDs for the profiling counters.
 enum { KID_STAGEA = 0, KID_STAGEB = 1, KID_FILLI = 2, KID_DTRED = 3, KID_STAT = 4, KID_MESH = 5 };
+// Host-side phases, wall milliseconds (slots 6, 7 of the same counters):
+// 6 = waiting for the step's flags + host admesh, 7 = list compile + upload.
+enum { HID_ADMESH = 6, HID_TABLES = 7 };
+
+struct HostTimer {
+  wx_ctx *c;
+  int slot;
+  std::chrono::steady_clock::time_point t0;
+  HostTimer(wx_ctx *c_, int slot_) : c(c_), slot(slot_), t0(std::chrono::steady_clock::now()) {}
+  ~HostTimer();
+};
 
 hipEvent_t take_event(wx_ctx *c) {
   if (!c->evpool.empty()) {
@@ -155,6 +167,12 @@ struct ProfScope {
     }
   }
 };
+
+HostTimer::~HostTimer() {
+  if (!c->prof) return;
+  c->prof_ms[slot] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  c->prof_n[slot] += 1;
+}
 
 void prof_collect(wx_ctx *c) {
   if (c->marks.empty()) return;
@@ -386,6 +404,7 @@ int upload_tables(wx_ctx *c) {
 
 int ensure_tables(wx_ctx *c) {
   if (c->tables_valid) return WX_OK;
+  HostTimer ht(c, HID_TABLES);
   if (!c->bounds_external) locatebounds(c);
   int rc = compile_tables(c);
   if (rc) return rc;
@@ -518,20 +537,35 @@ void updatelpdown(wx_ctx *c, int dad, int s1, int s2, int s3, int s4) {
 }
 
 // Runs the tree part of admesh from per-ID flags; records data ops.
-int admesh_tree(wx_ctx *c, std::vector<uint8_t> &up, std::vector<uint8_t> &dn, std::vector<MeshOp> &ops) {
+// The reference sweeps every block ID once per level (src/admesh.f90:29-105,
+// src/critneighup.f90:14-57, src/critneighdown.f90:14-50); only flagged
+// blocks ever act, so the sweeps here visit the flagged IDs, in ascending
+// ID order within each level as the reference does.
+int admesh_tree(wx_ctx *c, std::vector<uint8_t> &up, std::vector<uint8_t> &dn, std::vector<MeshOp> &ops,
+                std::vector<int> &upl, std::vector<int> &dnl) {
   const int nlevs = c->p.nlevs;
   int nbmaxp = c->nbmax;
   const int nbmaxp1 = c->nbmax;
-  for (int nl = nlevs - 1; nl >= 1; --nl)          // critneighup, src/critneighup.f90:14-57
-    for (int nb = 1; nb <= c->nbmax; ++nb)
-      if (up[nb] && c->LP(nb, 1) == nl)
-        for (int f = 5; f <= 8; ++f) {
-          const int n1 = c->LP(nb, f);
-          if (n1 != -1 && c->LP(n1, 1) < c->LP(nb, 1)) up[n1] = 1;
+  // critneighup: closure of "refine my coarser neighbours", finest level first
+  std::vector<std::vector<int>> byl((size_t)nlevs + 2);
+  for (int nb : upl) byl[c->LP(nb, 1)].push_back(nb);
+  for (int nl = nlevs - 1; nl >= 1; --nl)
+    for (size_t k = 0; k < byl[nl].size(); ++k) {
+      const int nb = byl[nl][k];
+      for (int f = 5; f <= 8; ++f) {
+        const int n1 = c->LP(nb, f);
+        if (n1 != -1 && c->LP(n1, 1) < c->LP(nb, 1) && !up[n1]) {
+          up[n1] = 1;
+          byl[c->LP(n1, 1)].push_back(n1);
         }
-  for (int nl = 1; nl <= nlevs - 1; ++nl)          // refine by level, src/admesh.f90:29-69
-    for (int nb = 1; nb <= nbmaxp1; ++nb) {
-      if (!(c->LP(nb, 1) == nl && up[nb])) continue;
+      }
+    }
+  for (int nl = 1; nl <= nlevs - 1; ++nl) {         // refine by level, src/admesh.f90:29-69
+    std::vector<int> &L = byl[nl];
+    std::sort(L.begin(), L.end());
+    L.erase(std::unique(L.begin(), L.end()), L.end());
+    for (int nb : L) {
+      if (nb > nbmaxp1 || !up[nb] || c->LP(nb, 1) != nl) continue;
       int s1, s2, s3, s4;
       if (c->LP(nb, 3) == 0) {
         s1 = nbmaxp + 1; s2 = s1 + 1; s3 = s1 + 2; s4 = s1 + 3;
@@ -546,14 +580,19 @@ int admesh_tree(wx_ctx *c, std::vector<uint8_t> &up, std::vector<uint8_t> &dn, s
       ops.push_back(MeshOp{0, nb, s1, s2, s3, s4, {0, 0}});
       updatelpup(c, nb, s1, s2, s3, s4, up, dn);
     }
-  for (int nb = 1; nb <= c->nbmax; ++nb)           // critneighdown, src/critneighdown.f90:14-50
-    if (dn[nb])
-      for (int f = 5; f <= 8; ++f) {
-        const int n1 = c->LP(nb, f);
-        if (n1 != -1 && !c->leaf[n1]) dn[nb] = 0;
-      }
-  for (int nl = nlevs; nl >= 2; --nl)              // coarsen by level, src/admesh.f90:75-105
-    for (int nb = 1; nb <= nbmaxp1; ++nb) {
+  }
+  std::vector<int> cand;
+  for (int nb : dnl) {                               // critneighdown, src/critneighdown.f90:14-50
+    if (!dn[nb]) continue;
+    for (int f = 5; f <= 8; ++f) {
+      const int n1 = c->LP(nb, f);
+      if (n1 != -1 && !c->leaf[n1]) dn[nb] = 0;
+    }
+    if (dn[nb] && nb <= nbmaxp1 && c->LP(nb, 4) == 1) cand.push_back(nb);
+  }
+  std::sort(cand.begin(), cand.end());
+  for (int nl = nlevs; nl >= 2; --nl)                // coarsen by level, src/admesh.f90:75-105
+    for (int nb : cand) {
       if (!(dn[nb] && c->LP(nb, 4) == 1 && c->LP(nb, 1) == nl)) continue;
       const int dad = c->LP(nb, 2), s1 = nb, s2 = c->LP(nb, 6), s3 = c->LP(nb, 8), s4 = c->LP(c->LP(nb, 6), 8);
       if (dn[s2] && dn[s3] && dn[s4]) {
@@ -561,6 +600,12 @@ int admesh_tree(wx_ctx *c, std::vector<uint8_t> &up, std::vector<uint8_t> &dn, s
         updatelpdown(c, dad, s1, s2, s3, s4);
       }
     }
+  // leave the scratch flags cleared for the next call
+  for (int nl = 0; nl <= nlevs; ++nl)
+    for (int nb : byl[nl]) up[nb] = 0;
+  for (const MeshOp &o : ops)
+    if (o.kind == 0) { up[o.a] = 0; dn[o.s1] = dn[o.s2] = dn[o.s3] = dn[o.s4] = 0; }
+  for (int nb : dnl) dn[nb] = 0;
   return WX_OK;
 }
 
@@ -596,6 +641,7 @@ int fetch_flags(wx_ctx *c) {
 int do_admesh(wx_ctx *c, int *changed) {
   int rc = ensure_stats(c);
   if (rc) return rc;
+  HostTimer ht(c, HID_ADMESH);
   rc = fetch_flags(c);
   if (rc) return rc;
   std::vector<uint8_t> &up = c->up, &dn = c->dn;
@@ -603,14 +649,13 @@ int do_admesh(wx_ctx *c, int *changed) {
     up.assign((size_t)c->cap + 1, 0);
     dn.assign((size_t)c->cap + 1, 0);
   }
+  std::vector<int> upl, dnl;
   for (const LeafRec &r : c->leaves) {   // markref, src/markref.f90:37-44
-    up[r.id] = c->hFlag[r.id] == 1;
-    dn[r.id] = c->hFlag[r.id] == 2;
+    if (c->hFlag[r.id] == 1) { up[r.id] = 1; upl.push_back(r.id); }
+    else if (c->hFlag[r.id] == 2) { dn[r.id] = 1; dnl.push_back(r.id); }
   }
   std::vector<MeshOp> ops;
-  rc = admesh_tree(c, up, dn, ops);
-  std::fill(up.begin(), up.begin() + c->nbmax + 1, 0);
-  std::fill(dn.begin(), dn.begin() + c->nbmax + 1, 0);
+  rc = admesh_tree(c, up, dn, ops, upl, dnl);
   if (rc) return rc;
   if (changed) *changed = (int)ops.size();
   if (!ops.empty()) {

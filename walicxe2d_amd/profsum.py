@@ -1,0 +1,97 @@
+"""Text summaries of rocprofv3 result databases (the `*_results.db` files
+that `rocprofv3 -d DIR -o NAME` writes).
+
+    python -m walicxe2d_amd.profsum kernels DB
+        per-kernel calls, total / average duration, share, VGPRs, LDS, scratch
+    python -m walicxe2d_amd.profsum traffic --copy-fetch DB --copy-write DB \
+            --fetch DB --write DB [--copy-mib 256]
+        HBM bytes per zone-update of the hot-path kernels from two
+        `--pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs, each counter scaled by
+        what it reads for a plain device copy of --copy-mib MiB per dispatch.
+
+A hot-path kernel launches one 256-thread workgroup per leaf block, so its
+grid size (threads) equals the interior cells it advances: the zone-updates
+of that launch.
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import json
+import re
+import sqlite3
+import sys
+
+HOT = ("k_stageA", "k_stageB", "k_fillI_cool", "k_ens_stageA", "k_ens_stageB", "k_ens_fillI_cool",
+       "k_cool_surrogate", "k_mlp_train_step")
+
+
+def short(name: str) -> str:
+    m = re.search(r"wx::(k_\w+(?:<[^>]*>)?)", name)
+    return m.group(1) if m else name[:48]
+
+
+def kernels(db: str) -> str:
+    cur = sqlite3.connect(db).cursor()
+    res = {}
+    for name, vg, ag, lds, scr in cur.execute(
+            "select name, max(vgpr_count), max(accum_vgpr_count), max(lds_size), max(scratch_size) "
+            "from kernels group by name"):
+        res[name] = (vg, ag, lds, scr)
+    rows = list(cur.execute("select name, total_calls, total_duration, average, percentage from top_kernels"))
+    out = ["| kernel | calls | total (ms) | average (us) | % | VGPR | AGPR | LDS (B) | scratch (B) |",
+           "|---|---|---|---|---|---|---|---|---|"]
+    for n, c, t, a, p in rows:
+        vg, ag, lds, scr = res.get(n, ("", "", "", ""))
+        out.append(f"| `{short(n)}` | {c} | {t / 1e3:.1f} | {a:.3f} | {p:.1f} | {vg} | {ag} | {lds} | {scr} |")
+    return "\n".join(out)
+
+
+def _per_kernel(db: str):
+    cur = sqlite3.connect(db).cursor()
+    acc = collections.defaultdict(lambda: [0.0, 0, 0])
+    for name, val, gs in cur.execute("select kernel_name, value, grid_size from counters_collection"):
+        a = acc[name]
+        a[0] += val
+        a[1] += 1
+        a[2] += gs
+    return acc
+
+
+def traffic(copy_fetch: str, copy_write: str, fetch: str, write: str, copy_mib: int) -> dict:
+    out = {"factors": {}, "bytes_per_zone_update": {}}
+    for ctr, cdb, kdb in (("FETCH_SIZE", copy_fetch, fetch), ("WRITE_SIZE", copy_write, write)):
+        cp = _per_kernel(cdb)
+        kib = [v[0] / v[1] for k, v in cp.items() if "copyBuffer" in k or "elementwise" in k.lower()]
+        if not kib:
+            raise SystemExit(f"no copy kernel in {cdb}")
+        fac = copy_mib * 1024 / max(kib)
+        out["factors"][ctr] = fac
+        for k, (tot, n, gs) in _per_kernel(kdb).items():
+            s = short(k).split("<")[0]
+            if s in HOT:
+                out["bytes_per_zone_update"].setdefault(s, {})[ctr] = tot * 1024 * fac / gs
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    k = sub.add_parser("kernels")
+    k.add_argument("db")
+    t = sub.add_parser("traffic")
+    t.add_argument("--copy-fetch", required=True)
+    t.add_argument("--copy-write", required=True)
+    t.add_argument("--fetch", required=True)
+    t.add_argument("--write", required=True)
+    t.add_argument("--copy-mib", type=int, default=256)
+    a = ap.parse_args(argv)
+    if a.cmd == "kernels":
+        print(kernels(a.db))
+    else:
+        print(json.dumps(traffic(a.copy_fetch, a.copy_write, a.fetch, a.write, a.copy_mib), indent=1))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
