@@ -156,9 +156,98 @@ int wx_profile(wx_ctx *ctx, int32_t enable, double *ms, int64_t *counts, int32_t
 
 /* Bookkeeping */
 int wx_info(wx_ctx *ctx, int32_t *nbmax, int32_t *nbleafs, int32_t *ninternal);
+
+/*
+ * Batched ensemble: n independent AMR meshes ("instances"), each with its own
+ * parameters (p[0..n-1], same neq), tree, time and timestep, advanced
+ * together.  The reference runs one mesh per process (the time loop
+ * src/main.f90:69-106); an ensemble replaces n such runs.  Each stage is one
+ * persistent launch over the compacted queue of every instance's leaf
+ * blocks; the step can be captured in a hipGraph; host work per instance
+ * (list compilation, admesh) runs on worker threads (WX_HOST_THREADS, else
+ * OMP_NUM_THREADS, at most 32).  Results of instance i are bit-identical to
+ * running it alone in a wx_ctx with the same parameters and initial data.
+ */
+typedef struct wx_ens wx_ens;
+int wx_ens_create(const wx_params *p, int32_t n, wx_ens **out);
+int wx_ens_destroy(wx_ens *ens);
+/* Root layout + initial data of instance inst (as wx_basegrid). */
+int wx_ens_basegrid(wx_ens *ens, int32_t inst, const double *uroots, int32_t nroots);
+/* admesh of every instance (src/admesh.f90:4-111), e.g. the initial
+ * refinement passes of src/main.f90:60-63.  Synchronising. */
+int wx_ens_admesh(wx_ens *ens, int32_t *changed);
+/* nsteps iterations of src/main.f90:87-98 for every instance (timestep,
+ * time += dt, tstep, admesh if do_admesh).  Returns after a device sync. */
+int wx_ens_step(wx_ens *ens, int32_t nsteps, int32_t do_admesh, double *zone_updates);
+int wx_ens_instance(wx_ens *ens, int32_t inst, double *time, double *dt, int32_t *nbmax, int32_t *nbleafs);
+/* Tiles 0..nbmax of instance inst, WX_LAYOUT_BLOCK. */
+int wx_ens_download(wx_ens *ens, int32_t inst, double *u);
+int wx_ens_get_tree(wx_ens *ens, int32_t inst, int32_t *lp, int32_t *leaf, int32_t *icoord, int32_t ld,
+                    int32_t *nbmax, int32_t *nbleafs);
+/* hipGraph capture of the step's launches (1 on, 0 off). */
+int wx_ens_set_graph(wx_ens *ens, int32_t enable);
+/* Workgroups of stage kernel k (0 stage A, 1 stage B, 2 boundaryI+cooling);
+ * 0 = one per queue entry. */
+int wx_ens_set_grid(wx_ens *ens, int32_t kernel, int32_t grid);
+/* Autotuner: times the three stage kernels at several launch shapes on the
+ * current meshes (reps launches each) and keeps the fastest; returns the
+ * chosen grids and their microseconds per launch.  Later steps give the same
+ * results as untuned ones. */
+int wx_ens_tune(wx_ens *ens, int32_t reps, int32_t *grids, double *us);
+int wx_ens_profile(wx_ens *ens, int32_t enable, double *ms, int64_t *counts, int32_t n);
+int wx_ens_info(wx_ens *ens, int32_t *ninst, int32_t *nleaf_total, int32_t *grids);
 int wx_last_error(char *buf, int32_t len);
 /* HIP stream the context launches on (hipStream_t as void*). */
 void *wx_stream(wx_ctx *ctx);
+
+/*
+ * Cooling surrogate.  No counterpart in the reference: a 4 -> 64 -> 64 ->
+ * 64 -> 2 SiLU network (bf16 weights and activations on the matrix cores,
+ * FP32 accumulation) stands in for coolingh's per-cell update
+ * (src/coolingh.f90:171-259) on states inside its training hull, and the
+ * exact routine runs wherever a state leaves the hull or the energy residual
+ * exceeds `tol`.  Features: log10 n, log10 T, x_HII = 1 - n_HI/n,
+ * log10(dt/t_cool) with t_cool from the routine's relaxation law
+ * (src/coolingh.f90:249-250) and a device table of ALOSS/n^2; outputs:
+ * c = log10(T1/T1_table), the correction to that relaxation law with the
+ * tabulated loss, and x_HII(new) - x_HII.  The energy residual is the
+ * change of thermal energy the correction makes, relative to the old one.
+ */
+typedef struct wx_sur_model {
+  float theta[8770];   /* W1[64][4], b1[64], W2[64][64], b2[64], W3[64][64], b3[64],
+                          W4[2][64], b4[2] (row-major [out][in]) */
+  float in_mu[4], in_sig[4];    /* feature standardisation */
+  float out_mu[2], out_sig[2];  /* output standardisation */
+  float lo[4], hi[4];           /* training hull of the raw features */
+  float tol;                    /* energy-residual tolerance (relative) */
+  int32_t pad[3];
+} wx_sur_model;
+
+/* Exact coolingh update of n states at rest, states[s] = (n, T, y0 =
+ * n_HI/n, dt in s), with the cooling parameters of p: out[s] = (T1, y1).
+ * These are the training pairs. */
+int wx_sur_pairs(const wx_params *p, const double *states, int64_t n, double *out);
+/* Raw features of n states (as above), feat[s][4], and ce*dt of the
+ * tabulated relaxation law, cedt[s]. */
+int wx_sur_features(const wx_params *p, const double *states, int64_t n, float *feat, float *cedt);
+/* Network on standardised inputs x[n][4] -> standardised outputs y[n][2]. */
+int wx_sur_infer(const wx_sur_model *m, const float *x, int64_t n, float *y);
+/* Mean over the n samples of w0 (y0 - t0)^2 + w1 (y1 - t1)^2 and its
+ * gradient with respect to theta (8770 floats). */
+int wx_sur_grad(const wx_sur_model *m, const float *x, const float *t, int64_t n, float w0, float w1,
+                float *loss, float *grad);
+/* `steps` Adam steps (lr decaying geometrically from lr0 to lr1) over
+ * consecutive batches of the n samples (wrapping); updates m->theta and
+ * writes each step's batch loss to loss_hist (may be NULL). */
+int wx_sur_train(wx_sur_model *m, const float *x, const float *t, int64_t n, int32_t batch, int32_t steps,
+                 float lr0, float lr1, float w0, float w1, float *loss_hist);
+/* Use the surrogate in the cooling pass of coolingh instances (m = NULL:
+ * exact cooling again), and read / reset the counts of cells it handled and
+ * of cells sent to the exact routine. */
+int wx_set_surrogate(wx_ctx *ctx, const wx_sur_model *m);
+int wx_ens_set_surrogate(wx_ens *ens, const wx_sur_model *m);
+int wx_sur_counters(wx_ctx *ctx, int32_t reset, int64_t *cells, int64_t *fallbacks);
+int wx_ens_sur_counters(wx_ens *ens, int32_t reset, int64_t *cells, int64_t *fallbacks);
 
 #ifdef __cplusplus
 }

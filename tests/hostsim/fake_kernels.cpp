@@ -4,7 +4,7 @@
 // the mesh-op launches record the ops they were given.  TEST SCAFFOLDING.
 #include <cstring>
 #include <vector>
-#include "../../walicxe2d_amd/csrc/wx_device.h"
+#include "../../walicxe2d_amd/csrc/wx_surrogate.h"
 
 static std::vector<signed char> g_flags;
 static std::vector<wx::MeshOp> g_ops;
@@ -29,28 +29,64 @@ static void write_flags(const LeafRec *leaves, int n, signed char *flag, double 
     stat[2 * id + 1] = 1e30;
   }
 }
-int launch_stageA(const wx_params *, int, const LeafRec *, int, const double *, double *, DevScalars *, void *) { return 0; }
-int launch_stageB(const wx_params *, int, const LeafRec *, int, const int2 *, const BlockRec *, double *,
-                  const double *, DevScalars *, void *) { return 0; }
-int launch_fillI_cool(const wx_params *, int, const LeafRec *leaves, int nleaf, const int2 *, const BlockRec *,
-                      const double *, double *, double *stat, int8_t *flag, DevScalars *, void *) {
-  write_flags(leaves, nleaf, (signed char *)flag, stat);
+int launch_build_queue(const QueueHdr *hdr, const int32_t *inst_off, const int32_t *leafbase,
+                       const LeafRec *reg_rec, const int2 *reg_ops2, const int2 *reg_ops1, LeafRec *rec,
+                       int2 *ops2, int2 *ops1, int, hipStream_t) {
+  for (int l = 0; l < hdr->nleaf; ++l) {
+    int i = 0;
+    while (i + 1 < hdr->ninst && inst_off[i + 1] <= l) ++i;
+    const size_t src = (size_t)leafbase[i] + (size_t)(l - inst_off[i]);
+    rec[l] = reg_rec[src];
+    for (int k = 0; k < 8; ++k) { ops2[(size_t)l * 8 + k] = reg_ops2[src * 8 + k]; ops1[(size_t)l * 8 + k] = reg_ops1[src * 8 + k]; }
+  }
+  return 0;
+}
+int launch_stageA(const wx_params *, int, QueueRef, int, const double *, double *, DevScalars *, hipStream_t) {
+  return 0;
+}
+int launch_stageB(const wx_params *, int, QueueRef, int, const int2 *, const BlockRec *, double *, const double *,
+                  DevScalars *, hipStream_t) {
+  return 0;
+}
+int launch_fillI_cool(const wx_params *, int, QueueRef q, int, const int2 *, const BlockRec *, const double *,
+                      double *, double *stat, int8_t *flag, DevScalars *, hipStream_t) {
+  write_flags(q.rec, q.hdr->nleaf, (signed char *)flag, stat);
   return 0;
 }
 int launch_blockstat(const wx_params *, int, const LeafRec *leaves, int nleaf, const double *, double *stat,
-                     int8_t *flag, void *) {
+                     int8_t *flag, hipStream_t) {
   write_flags(leaves, nleaf, (signed char *)flag, stat);
   return 0;
 }
-int launch_dtreduce(const wx_params *, const LeafRec *, const double *, int, DevScalars *S, int adv, void *) {
-  S->dt = 1e-3;
-  if (adv) S->time += S->dt;
+int launch_dtreduce(const wx_params *, const LeafRec *, const double *, const int32_t *, int ninst,
+                    DevScalars *S, int adv, hipStream_t) {
+  for (int i = 0; i < ninst; ++i) {
+    S[i].dt = 1e-3;
+    if (adv) S[i].time += S[i].dt;
+  }
   return 0;
 }
-int launch_meshops(const wx_params *, int, const MeshOp *ops, int nops, double *, void *) {
+int launch_gather_flags(QueueRef q, int, const int8_t *flag, int8_t *flagq, const DevScalars *S, DevScalars *Sout,
+                        int ninst, hipStream_t) {
+  for (int l = 0; l < q.hdr->nleaf; ++l) flagq[l] = flag[q.rec[l].id];
+  if (Sout) for (int i = 0; i < ninst; ++i) Sout[i] = S[i];
+  return 0;
+}
+int launch_meshops(int, const MeshOp *ops, int nops, double *, hipStream_t) {
   for (int k = 0; k < nops; ++k) g_ops.push_back(ops[k]);
   return 0;
 }
-int launch_transpose(int, int, const double *, double *, int, int, void *) { return 0; }
-int launch_calcprim(const wx_params *, int, int, const double *, double *, void *) { return 0; }
+int launch_transpose(int, int, const double *, double *, int, int, hipStream_t) { return 0; }
+int launch_fillI_cool_sur(const wx_params *P, int neq, QueueRef q, int grid, const int2 *ops, const BlockRec *binfo,
+                          const double *Uin, double *Uout, double *stat, int8_t *flag, DevScalars *S, SurDev *,
+                          hipStream_t st) {
+  return launch_fillI_cool(P, neq, q, grid, ops, binfo, Uin, Uout, stat, flag, S, st);
+}
+int sur_upload(SurDev *, const wx_sur_model *, const wx_params *, hipStream_t) { return 0; }
+int sur_counters(SurDev *, hipStream_t, int, int64_t *c, int64_t *f) {
+  if (c) *c = 0;
+  if (f) *f = 0;
+  return 0;
+}
+int launch_calcprim(const wx_params *, int, int, const double *, double *, hipStream_t) { return 0; }
 }  // namespace wx

@@ -1,0 +1,192 @@
+"""Cooling surrogate on the GPU (wx_sur_* through the C-ABI): the bf16
+matrix-core forward pass, the gradient and the Adam step against the CPU
+reference (oracle/mlp_ref.py); the training pairs against the oracle's
+coolingh; the loss table against the oracle's ALOSS; and the surrogate in
+the cooling jet against the exact routine."""
+import os
+
+import numpy as np
+import pytest
+
+import mlp_ref as M
+from walicxe2d_amd import params as P
+from walicxe2d_amd import surrogate as S
+from walicxe2d_amd.capi import Simulation
+from wxoracle import Oracle
+
+from harness import MASK
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_model(seed):
+    rng = np.random.default_rng(seed)
+    m = S.SurrogateModel(seed=seed)
+    m.theta = (m.theta + rng.normal(scale=0.05, size=m.theta.shape)).astype(np.float32)
+    return m
+
+
+def test_forward_matches_reference():
+    """MFMA forward (bf16 operands, FP32 accumulation) vs the numpy
+    restatement with the same bf16 roundings; a wrong operand layout or
+    transposed weight would give O(1) errors."""
+    m = _random_model(1)
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=(1000, 4)).astype(np.float32)   # 1000 = 62 groups of 16 + a partial group
+    y = S.infer(m, x)
+    yr, _ = M.forward(m.theta.astype(np.float64), x.astype(np.float64), round_bf16=True)
+    err = np.abs(y - yr)
+    scale = np.abs(yr).mean()
+    assert np.median(err) <= 1e-4 * scale, (np.median(err), scale)
+    assert err.max() <= 3e-2 * scale, (err.max(), scale)
+    # without rounding, the reference differs by the bf16 quantisation only
+    yf, _ = M.forward(m.theta.astype(np.float64), x.astype(np.float64), round_bf16=False)
+    assert np.abs(y - yf).max() <= 5e-2 * scale
+
+
+def test_gradient_matches_reference():
+    m = _random_model(3)
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=(777, 4)).astype(np.float32)
+    t = rng.normal(size=(777, 2)).astype(np.float32)
+    loss, g = S.grad(m, x, t, w=(1.0, 0.5))
+    lr, gr = M.loss_and_grad(m.theta.astype(np.float64), x.astype(np.float64), t.astype(np.float64),
+                             w=(1.0, 0.5), round_bf16=True)
+    assert abs(loss - lr) <= 1e-4 * lr
+    cos = float(g @ gr / (np.linalg.norm(g) * np.linalg.norm(gr)))
+    assert cos >= 0.9999, cos
+    assert np.abs(g - gr).max() <= 1e-2 * np.abs(gr).max()
+    # every parameter block gets a gradient (no block silently dropped)
+    k = 0
+    for o, i in M.SIZES:
+        for n in (o * i, o):
+            blk = slice(k, k + n)
+            assert np.linalg.norm(g[blk] - gr[blk]) <= 2e-2 * np.linalg.norm(gr[blk]) + 1e-12, (k, n)
+            k += n
+
+
+def test_adam_step_matches_reference():
+    m = _random_model(5)
+    rng = np.random.default_rng(6)
+    x = rng.normal(size=(512, 4)).astype(np.float32)
+    t = rng.normal(size=(512, 2)).astype(np.float32)
+    th0 = m.theta.copy()
+    _, g = S.grad(m, x, t)
+    S.train_steps(m, x, t, batch=512, steps=1, lr0=1e-3, lr1=1e-3)
+    th1, _, _ = M.adam(th0.astype(np.float64), g.astype(np.float64), np.zeros(M.NPARAM), np.zeros(M.NPARAM), 1, 1e-3)
+    assert np.allclose(m.theta, th1, rtol=1e-5, atol=2e-7)
+
+
+def test_training_reduces_loss():
+    """300 Adam steps on a smooth target halve the loss at least 10 times over."""
+    rng = np.random.default_rng(7)
+    x = rng.uniform(-2, 2, size=(8192, 4)).astype(np.float32)
+    t = np.stack([np.sin(x[:, 0]) * x[:, 1], np.tanh(x[:, 2] - x[:, 3])], axis=1).astype(np.float32)
+    m = S.SurrogateModel(seed=8)
+    hist = S.train_steps(m, x, t, batch=2048, steps=300, lr0=3e-3, lr1=3e-4)
+    assert hist[-20:].mean() < 0.1 * hist[:5].mean(), (hist[:5], hist[-20:])
+    y = S.infer(m, x)
+    assert np.mean((y - t) ** 2) < 0.1 * np.mean(t ** 2)
+
+
+def _states(n, seed):
+    rng = np.random.default_rng(seed)
+    return np.stack([10 ** rng.uniform(0.5, 3.5, n), 10 ** rng.uniform(1.5, 6.5, n), rng.uniform(1e-4, 1.0, n),
+                     10 ** rng.uniform(6, 9.5, n)], axis=1)
+
+
+def test_pairs_match_oracle_coolingh():
+    cfg = P.config_c3()
+    pp = cfg.params()
+    st = _states(400, 9)
+    out = S.exact_pairs(pp, st)
+    O = Oracle()
+    cv, tsc = pp.cv, pp.Tempsc
+    for s in range(st.shape[0]):
+        n, T, y0, dt = st[s]
+        uu = O.atomic(pp, dt, [n, 0.0, 0.0, cv * (2 * n - y0 * n) * T / tsc, y0 * n])
+        T1 = uu[3] / cv * tsc / (2 * uu[0] - uu[4])
+        y1 = uu[4] / uu[0]
+        assert abs(out[s, 0] - T1) <= 1e-11 * T1, (s, out[s], T1)
+        assert abs(out[s, 1] - y1) <= 1e-11, (s, out[s], y1)
+
+
+def test_loss_table_matches_oracle_aloss():
+    """ce*dt from the device table against the exact ALOSS (oracle), at
+    states with an unchanged neutral fraction: interpolation error only."""
+    cfg = P.config_c3()
+    pp = cfg.params()
+    st = _states(300, 10)
+    f, cedt = S.features(pp, st)
+    O = Oracle()
+    kb = 1.3807e-16
+    for s in range(st.shape[0]):
+        n, T, y0, dt = st[s]
+        al = O.aloss(y0, 0.0, 0.0, n, n * y0, T) / (n * n)
+        ce = 2 * n * al / (3 * kb * T) * dt
+        assert abs(cedt[s] - ce) <= 0.05 * ce + 1e-30, (s, cedt[s], ce)
+        assert abs(f[s, 0] - np.log10(n)) < 1e-5 and abs(f[s, 1] - np.log10(T)) < 1e-5
+        assert abs(f[s, 2] - (1 - y0)) < 1e-6
+
+
+MODEL = S.MODEL_PATH
+
+
+@pytest.mark.skipif(not os.path.exists(MODEL), reason="no trained model committed yet")
+def test_trained_model_beats_the_method_without_it():
+    """On states of a held-out jet variant, the network's corrections are
+    closer to the exact update than the tabulated relaxation law alone."""
+    model = S.SurrogateModel.load(MODEL)
+    cfg = P.config_c5(11, nbrootx=32, nbrooty=8, nblocks=12000)
+    st = S.harvest(cfg, 2400, 400, 40000, np.random.default_rng(11))
+    f, _, tgt = S.dataset(cfg.params(), st)
+    ev = S.errors(model, f, tgt)
+    print("held-out errors", ev)
+    assert ev["in_hull"] > 0.9
+    assert ev["rms_c"] < 0.5 * ev["rms_c_without"], ev
+    assert ev["rms_dx"] < 0.5 * ev["rms_dx_without"], ev
+
+
+@pytest.mark.skipif(not os.path.exists(MODEL), reason="no trained model committed yet")
+def test_surrogate_jet_density_error():
+    """The jet (reduced 32x8 roots, 4 levels) with the surrogate against the
+    exact routine over 1500 steps: relative L1 density error <= 1e-3 after
+    projection of both meshes to the finest grid."""
+    from walicxe2d_amd.l1 import l1_density
+    model = S.SurrogateModel.load(MODEL)
+    cfg = P.config_c3(nbrootx=32, nbrooty=8, nblocks=12000)
+    runs = []
+    for use in (False, True):
+        sim = Simulation(cfg)
+        sim.basegrid()
+        for _ in range(4):
+            sim.admesh()
+        if use:
+            sim.set_surrogate(model)
+        sim.step(1500)
+        u = sim.download()
+        lp, leaf, ic, _, _ = sim.tree()
+        cells, fb = sim.sur_counters()
+        runs.append((u, lp, leaf, ic, cells, fb))
+        sim.close()
+    err = l1_density(cfg, runs[1][:4], runs[0][:4])
+    cells, fb = runs[1][4], runs[1][5]
+    print(f"surrogate jet: L1 density error {err:.3e}, fallback fraction {fb / max(cells, 1):.3f}")
+    assert cells > 0
+    assert err <= 1e-3, err
+
+
+def test_surrogate_off_is_the_exact_path():
+    """Setting a model and removing it again leaves the plain path bit-exact."""
+    cfg = P.config_c3(nbrootx=16, nbrooty=4, nblocks=4000)
+    out = []
+    for toggle in (False, True):
+        sim = Simulation(cfg)
+        sim.basegrid()
+        if toggle:
+            sim.set_surrogate(_random_model(12))
+            sim.set_surrogate(None)
+        sim.step(5)
+        out.append(sim.download())
+        sim.close()
+    assert np.array_equal(out[0], out[1])

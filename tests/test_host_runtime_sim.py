@@ -19,3 +19,15 @@ def test_host_tree_matches_oracle_under_asan(seed):
                        timeout=600)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
     assert "trees identical" in r.stdout
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not installed")
+def test_ensemble_trees_match_oracle_under_asan():
+    """Three meshes of different root layouts, depths and capacities advanced as
+    one ensemble (worker threads on): every instance's tree after every admesh
+    equals the oracle's for the same flags."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    r = subprocess.run([os.path.join(HERE, "hostsim_main"), "120", "5", "ens"], capture_output=True, text=True,
+                       timeout=600, env={**os.environ, "WX_HOST_THREADS": "4"})
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    assert "trees identical" in r.stdout

@@ -28,6 +28,12 @@ DEVICE = [
 ]
 
 
+# device translation units
+KERNEL_SOURCES = ["wx_kernels.hip", "wx_surrogate.hip"]
+# host-side translation units
+HOST_SOURCES = ["wx_mesh.cpp", "wx_runtime.cpp", "wx_host.cpp", "wx_ensemble.cpp", "wx_surrogate.cpp"]
+
+
 def _run(cmd):
     print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
@@ -37,12 +43,14 @@ def build(verbose: bool = True) -> str:
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     objs = []
-    k_obj = os.path.join(objdir, "wx_kernels.o")
-    _run([HIPCC, *COMMON, *DEVICE, "-c", os.path.join(CSRC, "wx_kernels.hip"), "-o", k_obj])
-    objs.append(k_obj)
-    h_obj = os.path.join(objdir, "wx_host.o")
-    _run([HIPCC, *COMMON, *DEVICE, "-x", "hip", "-c", os.path.join(CSRC, "wx_host.cpp"), "-o", h_obj])
-    objs.append(h_obj)
+    for name in KERNEL_SOURCES:
+        k_obj = os.path.join(objdir, name + ".o")
+        _run([HIPCC, *COMMON, *DEVICE, "-c", os.path.join(CSRC, name), "-o", k_obj])
+        objs.append(k_obj)
+    for name in HOST_SOURCES:
+        h_obj = os.path.join(objdir, name + ".o")
+        _run([HIPCC, *COMMON, *DEVICE, "-x", "hip", "-c", os.path.join(CSRC, name), "-o", h_obj])
+        objs.append(h_obj)
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp, *objs])
     os.replace(tmp, LIB)

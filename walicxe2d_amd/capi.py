@@ -28,7 +28,22 @@ EXPORTS = [
     "wx_locatebounds", "wx_timestep", "wx_tstep", "wx_criteria", "wx_refine", "wx_coarse",
     "wx_upload_tree", "wx_basegrid", "wx_admesh", "wx_step", "wx_set_time", "wx_info",
     "wx_last_error", "wx_stream", "wx_profile",
+    "wx_ens_create", "wx_ens_destroy", "wx_ens_basegrid", "wx_ens_admesh", "wx_ens_step", "wx_ens_instance",
+    "wx_ens_download", "wx_ens_get_tree", "wx_ens_set_graph", "wx_ens_set_grid", "wx_ens_tune",
+    "wx_ens_profile", "wx_ens_info",
+    "wx_sur_pairs", "wx_sur_features", "wx_sur_infer", "wx_sur_grad", "wx_sur_train", "wx_set_surrogate",
+    "wx_ens_set_surrogate", "wx_sur_counters", "wx_ens_sur_counters",
 ]
+
+_fp = ctypes.POINTER(ctypes.c_float)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+class WxSurModel(ctypes.Structure):
+    """include/wx_capi.h wx_sur_model."""
+    _fields_ = [("theta", ctypes.c_float * 8770), ("in_mu", ctypes.c_float * 4), ("in_sig", ctypes.c_float * 4),
+                ("out_mu", ctypes.c_float * 2), ("out_sig", ctypes.c_float * 2), ("lo", ctypes.c_float * 4),
+                ("hi", ctypes.c_float * 4), ("tol", ctypes.c_float), ("pad", ctypes.c_int32 * 3)]
 
 KERNELS = ["stageA", "stageB", "fillI_cool", "dtreduce", "blockstat", "meshops", "host_admesh", "host_tables"]
 
@@ -69,6 +84,30 @@ def load_library(path: str = LIBPATH) -> ctypes.CDLL:
     L.wx_stream.argtypes = [P]
     L.wx_stream.restype = P
     L.wx_profile.argtypes = [P, ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]
+    L.wx_ens_create.argtypes = [ctypes.POINTER(WxParams), ctypes.c_int32, ctypes.POINTER(P)]
+    L.wx_ens_destroy.argtypes = [P]
+    L.wx_ens_basegrid.argtypes = [P, ctypes.c_int32, _dp, ctypes.c_int32]
+    L.wx_ens_admesh.argtypes = [P, _ip]
+    L.wx_ens_step.argtypes = [P, ctypes.c_int32, ctypes.c_int32, _dp]
+    L.wx_ens_instance.argtypes = [P, ctypes.c_int32, _dp, _dp, _ip, _ip]
+    L.wx_ens_download.argtypes = [P, ctypes.c_int32, _dp]
+    L.wx_ens_get_tree.argtypes = [P, ctypes.c_int32, _ip, _ip, _ip, ctypes.c_int32, _ip, _ip]
+    L.wx_ens_set_graph.argtypes = [P, ctypes.c_int32]
+    L.wx_ens_set_grid.argtypes = [P, ctypes.c_int32, ctypes.c_int32]
+    L.wx_ens_tune.argtypes = [P, ctypes.c_int32, _ip, _dp]
+    L.wx_ens_profile.argtypes = [P, ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]
+    L.wx_ens_info.argtypes = [P, _ip, _ip, _ip]
+    SM = ctypes.POINTER(WxSurModel)
+    L.wx_sur_pairs.argtypes = [ctypes.POINTER(WxParams), _dp, ctypes.c_int64, _dp]
+    L.wx_sur_features.argtypes = [ctypes.POINTER(WxParams), _dp, ctypes.c_int64, _fp, _fp]
+    L.wx_sur_infer.argtypes = [SM, _fp, ctypes.c_int64, _fp]
+    L.wx_sur_grad.argtypes = [SM, _fp, _fp, ctypes.c_int64, ctypes.c_float, ctypes.c_float, _fp, _fp]
+    L.wx_sur_train.argtypes = [SM, _fp, _fp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                               ctypes.c_float, ctypes.c_float, ctypes.c_float, _fp]
+    L.wx_set_surrogate.argtypes = [P, SM]
+    L.wx_ens_set_surrogate.argtypes = [P, SM]
+    L.wx_sur_counters.argtypes = [P, ctypes.c_int32, _i64p, _i64p]
+    L.wx_ens_sur_counters.argtypes = [P, ctypes.c_int32, _i64p, _i64p]
     _lib = L
     return L
 
@@ -235,9 +274,136 @@ class Simulation:
                "wx_profile")
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(KERNELS)}
 
+    def set_surrogate(self, model):
+        """Cooling surrogate (a walicxe2d_amd.surrogate.SurrogateModel) in the
+        coolingh pass; None restores the exact routine everywhere."""
+        _check(self.L.wx_set_surrogate(self.h, ctypes.byref(model.to_c()) if model is not None else None),
+               "wx_set_surrogate")
+
+    def sur_counters(self, reset: bool = False):
+        """(cells the surrogate pass handled, cells sent to the exact routine)."""
+        c, f = ctypes.c_int64(), ctypes.c_int64()
+        _check(self.L.wx_sur_counters(self.h, 1 if reset else 0, ctypes.byref(c), ctypes.byref(f)),
+               "wx_sur_counters")
+        return c.value, f.value
+
     def step(self, nsteps: int = 1, admesh: bool = True):
         """nsteps iterations of src/main.f90:87-98; returns (time, dt, zone_updates)."""
         t, dt, z = np.zeros(1), np.zeros(1), np.zeros(1)
         _check(self.L.wx_step(self.h, nsteps, 1 if admesh else 0, _d(t), _d(dt), _d(z)), "wx_step")
         self.time = float(t[0])
         return float(t[0]), float(dt[0]), float(z[0])
+
+
+class Ensemble:
+    """Many independent AMR meshes (one Config each, same neq) advanced
+    together on the GPU behind the wx_ens_* entry points: each stage is one
+    persistent launch over all instances' leaf blocks."""
+
+    def __init__(self, cfgs):
+        self.L = load_library()
+        self.cfgs = list(cfgs)
+        self.n = len(self.cfgs)
+        if self.n < 1:
+            raise WxError("an ensemble needs at least one instance")
+        self.neq = self.cfgs[0].neq
+        arr = (WxParams * self.n)(*[c.params() for c in self.cfgs])
+        h = ctypes.c_void_p()
+        _check(self.L.wx_ens_create(arr, self.n, ctypes.byref(h)), "wx_ens_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.wx_ens_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def basegrid(self):
+        for i, c in enumerate(self.cfgs):
+            u, lp, _ = initial_state(c)
+            _check(self.L.wx_ens_basegrid(self.h, i, _d(np.ascontiguousarray(u)), lp.shape[0] - 1),
+                   "wx_ens_basegrid")
+
+    def admesh(self) -> int:
+        ch = ctypes.c_int32()
+        _check(self.L.wx_ens_admesh(self.h, ctypes.byref(ch)), "wx_ens_admesh")
+        return ch.value
+
+    def step(self, nsteps: int = 1, admesh: bool = True) -> float:
+        """nsteps iterations for every instance; returns the zone-updates."""
+        z = np.zeros(1)
+        _check(self.L.wx_ens_step(self.h, nsteps, 1 if admesh else 0, _d(z)), "wx_ens_step")
+        return float(z[0])
+
+    def instance(self, i: int):
+        """(time, dt, nbmax, nbleafs) of instance i."""
+        t, dt = np.zeros(1), np.zeros(1)
+        a, b = ctypes.c_int32(), ctypes.c_int32()
+        _check(self.L.wx_ens_instance(self.h, i, _d(t), _d(dt), ctypes.byref(a), ctypes.byref(b)),
+               "wx_ens_instance")
+        return float(t[0]), float(dt[0]), a.value, b.value
+
+    def download(self, i: int):
+        nbmax = self.instance(i)[2]
+        u = np.zeros((nbmax + 1, self.neq, NP, NP))
+        _check(self.L.wx_ens_download(self.h, i, _d(u)), "wx_ens_download")
+        return u
+
+    def tree(self, i: int):
+        nbmax = self.instance(i)[2]
+        ld = max(nbmax, 1)
+        flp = np.zeros((8, ld), dtype=np.int32)
+        fleaf = np.zeros(ld, dtype=np.int32)
+        fic = np.zeros((2, ld), dtype=np.int32)
+        a, b = ctypes.c_int32(), ctypes.c_int32()
+        _check(self.L.wx_ens_get_tree(self.h, i, _i(flp), _i(fleaf), _i(fic), ld, ctypes.byref(a),
+                                      ctypes.byref(b)), "wx_ens_get_tree")
+        lp = np.zeros((ld + 1, 8), dtype=np.int32)
+        leaf = np.zeros(ld + 1, dtype=np.int32)
+        icoord = np.zeros((ld + 1, 2), dtype=np.int32)
+        lp[1:] = flp.T
+        leaf[1:] = fleaf
+        icoord[1:] = fic.T
+        return lp, leaf, icoord, a.value, b.value
+
+    def set_graph(self, on: bool):
+        _check(self.L.wx_ens_set_graph(self.h, 1 if on else 0), "wx_ens_set_graph")
+
+    def set_grid(self, kernel: int, grid: int):
+        _check(self.L.wx_ens_set_grid(self.h, kernel, grid), "wx_ens_set_grid")
+
+    def tune(self, reps: int = 5):
+        """Autotunes the stage kernels' launch shapes; returns (grids, us per launch)."""
+        g = np.zeros(3, dtype=np.int32)
+        us = np.zeros(3)
+        _check(self.L.wx_ens_tune(self.h, reps, _i(g), _d(us)), "wx_ens_tune")
+        return [int(x) for x in g], [float(x) for x in us]
+
+    def info(self):
+        """(instances, leaf blocks in the queue, stage grids)."""
+        a, b = ctypes.c_int32(), ctypes.c_int32()
+        g = np.zeros(3, dtype=np.int32)
+        _check(self.L.wx_ens_info(self.h, ctypes.byref(a), ctypes.byref(b), _i(g)), "wx_ens_info")
+        return a.value, b.value, [int(x) for x in g]
+
+    def set_surrogate(self, model):
+        _check(self.L.wx_ens_set_surrogate(self.h, ctypes.byref(model.to_c()) if model is not None else None),
+               "wx_ens_set_surrogate")
+
+    def sur_counters(self, reset: bool = False):
+        c, f = ctypes.c_int64(), ctypes.c_int64()
+        _check(self.L.wx_ens_sur_counters(self.h, 1 if reset else 0, ctypes.byref(c), ctypes.byref(f)),
+               "wx_ens_sur_counters")
+        return c.value, f.value
+
+    def profile(self, enable: int = -1):
+        ms = np.zeros(8)
+        n = np.zeros(8, dtype=np.int64)
+        _check(self.L.wx_ens_profile(self.h, enable, _d(ms), n.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 8),
+               "wx_ens_profile")
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(KERNELS)}

@@ -30,14 +30,35 @@ constexpr int GF_SRC_IMPOSED = 1 << 4;   // source interior carried the jet at c
 constexpr int GF_DST_IMPOSED = 1 << 5;   // destination imposed after this write
 constexpr int GF_OFF_SHIFT = 8;          // PROLONG: 0 or 8 along the coarse side
 
-// Per-leaf record (one workgroup each).
+// Per-leaf record: one entry of the block queue.  A workgroup takes queue
+// entries l = blockIdx.x, blockIdx.x + gridDim.x, ... while l < nleaf.
 struct LeafRec {
-  int32_t id;      // block ID (tile slot)
+  int32_t id;      // tile slot (the block ID; + the instance's slot base in an ensemble)
   int32_t level;
   int32_t icx, icy;
   int32_t flags;   // bit0: block imposed by the jet source in this step's fills
-  int32_t lidx;    // position in the leaf list
+  int32_t lidx;    // position in the queue
+  int32_t inst;    // instance: selects wx_params and DevScalars
+  int32_t pad;
+};
+
+// Queue header in device memory.  Kernels read the leaf count from here, so
+// launches (and a captured graph of them) stay valid when the mesh changes.
+struct QueueHdr {
+  int32_t nleaf;
+  int32_t ninst;
   int32_t pad[2];
+};
+
+// The compacted block queue as the kernels see it: entries 0 .. hdr->nleaf-1
+// of rec (capacity `cap` entries, cap >= 1); the 8 ghost ops per fill of
+// entry l are at ops[8 * l].  In an ensemble, k_build_queue gathers the
+// instances' record and op regions into these compact arrays after a mesh
+// change, so only the regions of instances whose mesh changed are uploaded.
+struct QueueRef {
+  const QueueHdr *hdr;
+  const LeafRec *rec;
+  int32_t cap;
 };
 
 // Per-ID record for source lookups (level and lower-left cell index).
@@ -62,24 +83,35 @@ struct MeshOp {
   int32_t pad[2];
 };
 
-// launch wrappers (wx_kernels.hip)
-int launch_stageA(const wx_params *dP, int neq, const LeafRec *leaves, int nleaf,
-                  const double *U, double *Q, DevScalars *S, void *stream);
-int launch_stageB(const wx_params *dP, int neq, const LeafRec *leaves, int nleaf,
-                  const int2 *ops, const BlockRec *binfo, double *U, const double *Q,
-                  DevScalars *S, void *stream);
-int launch_fillI_cool(const wx_params *dP, int neq, const LeafRec *leaves, int nleaf,
-                      const int2 *ops, const BlockRec *binfo, const double *Uin, double *Uout,
-                      double *stat, int8_t *flag, DevScalars *S, void *stream);
-int launch_blockstat(const wx_params *dP, int neq, const LeafRec *leaves, int nleaf,
-                     const double *U, double *stat, int8_t *flag, void *stream);
-int launch_dtreduce(const wx_params *dP, const LeafRec *leaves, const double *stat, int nleaf,
-                    DevScalars *S, int advance_time, void *stream);
-int launch_meshops(const wx_params *dP, int neq, const MeshOp *ops, int nops, double *U,
-                   void *stream);
-int launch_transpose(int neq, int cap, const double *src, double *dst, int to_fortran,
-                     int ld, void *stream);
-int launch_calcprim(const wx_params *dP, int neq, int cap, const double *U, double *prim,
-                    void *stream);
+// launch wrappers (wx_kernels.hip).  P and S are arrays indexed by
+// LeafRec::inst (one element for a single mesh); `grid` is the number of
+// workgroups of the persistent stage kernels.
+// Compacts the per-instance regions (records and both op tables, instance i
+// at region offset leafbase[i], inst_off[i+1] - inst_off[i] entries) into
+// the queue arrays in instance order.
+int launch_build_queue(const QueueHdr *hdr, const int32_t *inst_off, const int32_t *leafbase,
+                       const LeafRec *reg_rec, const int2 *reg_ops2, const int2 *reg_ops1, LeafRec *rec,
+                       int2 *ops2, int2 *ops1, int grid, hipStream_t stream);
+int launch_stageA(const wx_params *P, int neq, QueueRef q, int grid, const double *U, double *Q, DevScalars *S,
+                  hipStream_t stream);
+int launch_stageB(const wx_params *P, int neq, QueueRef q, int grid, const int2 *ops, const BlockRec *binfo,
+                  double *U, const double *Q, DevScalars *S, hipStream_t stream);
+int launch_fillI_cool(const wx_params *P, int neq, QueueRef q, int grid, const int2 *ops, const BlockRec *binfo,
+                      const double *Uin, double *Uout, double *stat, int8_t *flag, DevScalars *S,
+                      hipStream_t stream);
+// Statistics of an explicit list of leaf records (one workgroup each).
+int launch_blockstat(const wx_params *P, int neq, const LeafRec *leaves, int nleaf, const double *U,
+                     double *stat, int8_t *flag, hipStream_t stream);
+// Workgroup i reduces the CFL partials of instance i's queue entries
+// [inst_off[i], inst_off[i+1]).
+int launch_dtreduce(const wx_params *P, const LeafRec *rec, const double *stat, const int32_t *inst_off,
+                    int ninst, DevScalars *S, int advance_time, hipStream_t stream);
+// flagq[l] = flag[rec[l].id] for the queue; Sout[0..ninst) = S (if Sout).
+int launch_gather_flags(QueueRef q, int grid, const int8_t *flag, int8_t *flagq, const DevScalars *S,
+                        DevScalars *Sout, int ninst, hipStream_t stream);
+int launch_meshops(int neq, const MeshOp *ops, int nops, double *U, hipStream_t stream);
+int launch_transpose(int neq, int cap, const double *src, double *dst, int to_fortran, int ld,
+                     hipStream_t stream);
+int launch_calcprim(const wx_params *P, int neq, int cap, const double *U, double *prim, hipStream_t stream);
 
 }  // namespace wx

@@ -1,0 +1,620 @@
+// wx_ensemble.cpp -- many independent AMR meshes ("instances") advanced
+// together behind the wx_ens_* entry points of include/wx_capi.h.
+//
+// Every instance keeps its own host tree, ghost list, parameters, time and
+// timestep (the reference's whole time loop, src/main.f90:69-106, run once
+// per instance).  On the device all instances share one slot space: the
+// block ID `id` of instance i lives in tile slot slotbase[i] + id, and its
+// leaf records and ghost ops in the region starting at leafbase[i].  After a
+// mesh change only the regions of instances whose mesh changed are
+// uploaded, and k_build_queue gathers all regions into the compacted queue.
+// Each stage is ONE persistent launch over that queue.  The step's
+// launches (dt reduction, the three stages, the flag gather) can be
+// captured in two hipGraphs (one per tile-buffer parity); as the kernels
+// read the queue length from device memory and the buffers are allocated at
+// full capacity up front, the graphs never need re-capturing when meshes
+// change.  Per-instance host work (table compilation, admesh) runs on a
+// pool of worker threads.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "wx_runtime.h"
+#include "wx_surrogate.h"
+
+using namespace wx;
+
+struct wx_ens : Runtime {
+  int n = 0, neq = 0;
+  std::vector<Mesh> m;
+  std::vector<int32_t> slotbase, leafbase;   // per instance
+  size_t nslots = 0, nrec = 0;
+  int ncu = 256;
+  // device state
+  DevBuf<double> U[2];
+  int cur = 0;
+  DevBuf<wx_params> dP;
+  DevBuf<DevScalars> dS;
+  DevBuf<QueueHdr> dQ;
+  DevBuf<int32_t> dInstOff, dLeafBase;
+  DevBuf<LeafRec> dRegRec, dRec;          // per-instance regions, compacted queue
+  DevBuf<BlockRec> dBinfo;
+  DevBuf<int2> dRegOpsII, dRegOpsI, dOpsII, dOpsI;
+  DevBuf<double> dStat;
+  DevBuf<int8_t> dFlag;
+  DevBuf<MeshOp> dMeshOps;
+  DevBuf<LeafRec> dDirty;
+  // mapped pinned mirrors written by k_gather_flags
+  DevScalars *hS = nullptr, *hS_dev = nullptr;
+  int8_t *hFlagQ = nullptr, *hFlagQ_dev = nullptr;
+  // queue bookkeeping
+  std::vector<int32_t> inst_off;   // n + 1 prefix sums of leaf counts
+  int nleaf_total = 0;
+  // host scratch
+  std::vector<int> rcs;
+  std::vector<std::string> errs;
+  std::vector<std::vector<MeshOp>> iops;
+  std::vector<MeshOp> ops;
+  std::vector<LeafRec> dirty;
+  std::unique_ptr<WorkerPool> pool;
+  // launch shapes: workgroups of the persistent stage kernels (A, B, fillI);
+  // 0 = one workgroup per queue entry (eager launches only)
+  int grid[3] = {0, 0, 0};
+  // cooling surrogate for the coolingh instances, used when use_sur
+  DevBuf<SurDev> dSur;
+  bool use_sur = false;
+  // graphs, one per tile-buffer parity
+  bool use_graph = false;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+
+  double *cur_u() { return U[cur].ptr; }
+  double *other_u() { return U[cur ^ 1].ptr; }
+  QueueRef queue() const { return QueueRef{dQ.ptr, dRec.ptr, (int32_t)dRec.n}; }
+  size_t tile() const { return (size_t)neq * NC; }
+};
+
+namespace {
+
+int collect_errors(wx_ens *e, const std::vector<int> &which) {
+  for (int i : which)
+    if (e->rcs[i]) return fail(e->rcs[i], "instance %d: %s", i, e->errs[i].c_str());
+  return WX_OK;
+}
+
+void drop_graphs(wx_ens *e) {
+  for (int k = 0; k < 2; ++k) {
+    if (e->gexec[k]) (void)hipGraphExecDestroy(e->gexec[k]);
+    if (e->graph[k]) (void)hipGraphDestroy(e->graph[k]);
+    e->gexec[k] = nullptr;
+    e->graph[k] = nullptr;
+  }
+}
+
+// Grid of stage kernel k for the current queue.
+int stage_grid(const wx_ens *e, int k, bool for_graph) {
+  if (e->grid[k] > 0) return e->grid[k];
+  if (for_graph) return e->ncu * (k == 2 ? 8 : 4);   // LDS-limited residency per CU
+  return e->nleaf_total;
+}
+
+// Recompiles the tables of instances whose mesh changed and uploads their
+// regions, the leaf-count prefix sums, and rebuilds the queue index.
+int ensure_queue(wx_ens *e) {
+  std::vector<int> stale;
+  for (int i = 0; i < e->n; ++i)
+    if (!e->m[i].tables_valid) stale.push_back(i);
+  if (stale.empty()) return WX_OK;
+  HostTimer ht(e, HID_TABLES);
+  e->pool->run((int)stale.size(), [&](int k) {
+    const int i = stale[k];
+    e->rcs[i] = refresh_tables(&e->m[i]);
+    if (e->rcs[i]) e->errs[i] = last_error();
+  });
+  int rc = collect_errors(e, stale);
+  if (rc) return rc;
+  e->inst_off[0] = 0;
+  for (int i = 0; i < e->n; ++i) e->inst_off[i + 1] = e->inst_off[i] + (int32_t)e->m[i].leaves.size();
+  e->nleaf_total = e->inst_off[e->n];
+  size_t total = al256(sizeof(QueueHdr)) + al256((size_t)(e->n + 1) * sizeof(int32_t));
+  for (int i : stale) {
+    const size_t nl = e->m[i].leaves.size();
+    total += al256(nl * sizeof(LeafRec)) + 2 * al256(nl * 8 * sizeof(int2)) + al256(e->m[i].binfo.size() * sizeof(BlockRec));
+  }
+  if ((rc = stage_begin(e, total))) return rc;
+  struct Chunk { char *rec, *o2, *o1, *bi; };
+  std::vector<Chunk> ch(stale.size());
+  for (size_t k = 0; k < stale.size(); ++k) {
+    const Mesh &m = e->m[stale[k]];
+    const size_t nl = m.leaves.size();
+    ch[k].rec = stage_take(e, nl * sizeof(LeafRec));
+    ch[k].o2 = stage_take(e, nl * 8 * sizeof(int2));
+    ch[k].o1 = stage_take(e, nl * 8 * sizeof(int2));
+    ch[k].bi = stage_take(e, m.binfo.size() * sizeof(BlockRec));
+  }
+  e->pool->run((int)stale.size(), [&](int k) {   // local IDs -> global slots
+    const int i = stale[k];
+    const Mesh &m = e->m[i];
+    const int sb = e->slotbase[i];
+    LeafRec *r = (LeafRec *)ch[k].rec;
+    for (size_t l = 0; l < m.leaves.size(); ++l) {
+      r[l] = m.leaves[l];
+      r[l].id += sb;
+      r[l].inst = i;
+    }
+    int2 *o2 = (int2 *)ch[k].o2, *o1 = (int2 *)ch[k].o1;
+    for (size_t t = 0; t < m.opsII.size(); ++t) {
+      o2[t] = int2{m.opsII[t].x + sb, m.opsII[t].y};
+      o1[t] = int2{m.opsI[t].x + sb, m.opsI[t].y};
+    }
+    memcpy(ch[k].bi, m.binfo.data(), m.binfo.size() * sizeof(BlockRec));
+  });
+  for (size_t k = 0; k < stale.size(); ++k) {
+    const int i = stale[k];
+    const Mesh &m = e->m[i];
+    const size_t nl = m.leaves.size();
+    if ((rc = stage_push(e, e->dRegRec.ptr + e->leafbase[i], ch[k].rec, nl * sizeof(LeafRec)))) return rc;
+    if ((rc = stage_push(e, e->dRegOpsII.ptr + (size_t)e->leafbase[i] * 8, ch[k].o2, nl * 8 * sizeof(int2)))) return rc;
+    if ((rc = stage_push(e, e->dRegOpsI.ptr + (size_t)e->leafbase[i] * 8, ch[k].o1, nl * 8 * sizeof(int2)))) return rc;
+    if ((rc = stage_push(e, e->dBinfo.ptr + e->slotbase[i], ch[k].bi, m.binfo.size() * sizeof(BlockRec)))) return rc;
+  }
+  const QueueHdr qh{e->nleaf_total, e->n, {0, 0}};
+  if ((rc = stage_copy(e, e->dQ.ptr, &qh, sizeof qh))) return rc;
+  if ((rc = stage_copy(e, e->dInstOff.ptr, e->inst_off.data(), (size_t)(e->n + 1) * sizeof(int32_t)))) return rc;
+  stage_end(e);
+  rc = launch_build_queue(e->dQ.ptr, e->dInstOff.ptr, e->dLeafBase.ptr, e->dRegRec.ptr, e->dRegOpsII.ptr,
+                          e->dRegOpsI.ptr, e->dRec.ptr, e->dOpsII.ptr, e->dOpsI.ptr,
+                          std::max(1, std::min((8 * e->nleaf_total + 255) / 256, 4096)), e->stream);
+  if (rc) return fail(rc, "queue build launch failed");
+  return WX_OK;
+}
+
+// Block statistics of new leaves (or of every leaf of an instance whose
+// statistics are all stale), one launch for all instances.
+int ensure_stats(wx_ens *e) {
+  int rc = ensure_queue(e);
+  if (rc) return rc;
+  e->dirty.clear();
+  for (int i = 0; i < e->n; ++i) {
+    Mesh &m = e->m[i];
+    const int sb = e->slotbase[i];
+    auto add = [&](const LeafRec &r0) {
+      LeafRec r = r0;
+      r.id += sb;
+      r.inst = i;
+      e->dirty.push_back(r);
+    };
+    if (m.stats_all_stale) {
+      for (const LeafRec &r : m.leaves) add(r);
+      m.stats_all_stale = false;
+    } else {
+      for (int id : m.new_ids)
+        if (id >= 1 && id <= m.nbmax && m.lidx[id] >= 0) add(m.leaves[m.lidx[id]]);
+    }
+    m.new_ids.clear();
+  }
+  if (e->dirty.empty()) return WX_OK;
+  if ((rc = e->dDirty.ensure(e->dirty.size()))) return rc;
+  if ((rc = stage_begin(e, al256(e->dirty.size() * sizeof(LeafRec))))) return rc;
+  if ((rc = stage_copy(e, e->dDirty.ptr, e->dirty.data(), e->dirty.size() * sizeof(LeafRec)))) return rc;
+  stage_end(e);
+  ProfScope ps(e, KID_STAT);
+  rc = launch_blockstat(e->dP.ptr, e->neq, e->dDirty.ptr, (int)e->dirty.size(), e->cur_u(), e->dStat.ptr,
+                        e->dFlag.ptr, e->stream);
+  if (rc) return fail(rc, "blockstat launch failed");
+  return WX_OK;
+}
+
+// dt reduction (time += dt), the three stages and the flag gather, reading
+// U[par] and writing U[par ^ 1]; `prof` records kernel events.
+int launch_step(wx_ens *e, int par, bool for_graph) {
+  const bool prof = !for_graph;
+  double *Ucur = e->U[par].ptr, *Uoth = e->U[par ^ 1].ptr;
+  const QueueRef q = e->queue();
+  int rc;
+  {
+    ProfScope ps(e, prof ? KID_DTRED : -1);
+    rc = launch_dtreduce(e->dP.ptr, e->dRec.ptr, e->dStat.ptr, e->dInstOff.ptr, e->n, e->dS.ptr, 1, e->stream);
+  }
+  if (!rc) {
+    ProfScope ps(e, prof ? KID_STAGEA : -1);
+    rc = launch_stageA(e->dP.ptr, e->neq, q, stage_grid(e, 0, for_graph), Ucur, Uoth, e->dS.ptr, e->stream);
+  }
+  if (!rc) {
+    ProfScope ps(e, prof ? KID_STAGEB : -1);
+    rc = launch_stageB(e->dP.ptr, e->neq, q, stage_grid(e, 1, for_graph), e->dOpsII.ptr, e->dBinfo.ptr, Ucur, Uoth,
+                       e->dS.ptr, e->stream);
+  }
+  if (!rc) {
+    ProfScope ps(e, prof ? KID_FILLI : -1);
+    if (e->use_sur)
+      rc = launch_fillI_cool_sur(e->dP.ptr, e->neq, q, stage_grid(e, 2, for_graph), e->dOpsI.ptr, e->dBinfo.ptr, Ucur,
+                                 Uoth, e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, e->dSur.ptr, e->stream);
+    else
+      rc = launch_fillI_cool(e->dP.ptr, e->neq, q, stage_grid(e, 2, for_graph), e->dOpsI.ptr, e->dBinfo.ptr, Ucur,
+                             Uoth, e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, e->stream);
+  }
+  if (!rc) {
+    const int g = for_graph ? std::max(1, (int)std::min<size_t>((e->nrec + 255) / 256, 1024))
+                            : std::max(1, std::min((e->nleaf_total + 255) / 256, 1024));
+    rc = launch_gather_flags(q, g, e->dFlag.ptr, e->hFlagQ_dev, e->dS.ptr, e->hS_dev, e->n, e->stream);
+  }
+  if (rc) return fail(rc, "step launch failed: %s", hipGetErrorString(hipGetLastError()));
+  return WX_OK;
+}
+
+int ensure_graphs(wx_ens *e) {
+  if (e->gexec[0] && e->gexec[1]) return WX_OK;
+  drop_graphs(e);
+  for (int par = 0; par < 2; ++par) {
+    HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_step(e, par, true);
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(e->stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (ce != hipSuccess) return fail(WX_EHIP, "graph capture failed: %s", hipGetErrorString(ce));
+    e->graph[par] = g;
+    HIPCHK(hipGraphInstantiate(&e->gexec[par], g, nullptr, nullptr, 0));
+  }
+  return WX_OK;
+}
+
+int check_scalars(wx_ens *e) {
+  for (int i = 0; i < e->n; ++i)
+    if (e->hS[i].err)
+      return fail(WX_EHLLC, "Error in hllc (instance %d, device flag %d): a Riemann problem had no valid star speed", i,
+                  e->hS[i].err);
+  return WX_OK;
+}
+
+// admesh of every instance from the flags the last gather wrote (queue order).
+int admesh_all(wx_ens *e, int *changed) {
+  HostTimer ht(e, HID_ADMESH);
+  HIPCHK(hipStreamSynchronize(e->stream));
+  int rc = check_scalars(e);
+  if (rc) return rc;
+  std::vector<int> all(e->n);
+  for (int i = 0; i < e->n; ++i) all[i] = i;
+  e->pool->run(e->n, [&](int i) {
+    e->iops[i].clear();
+    if (e->m[i].p.nlevs <= 1) { e->rcs[i] = 0; return; }
+    e->rcs[i] = admesh_from_flags(&e->m[i], e->hFlagQ + e->inst_off[i], e->iops[i]);
+    if (e->rcs[i]) e->errs[i] = last_error();
+  });
+  rc = collect_errors(e, all);
+  if (rc) return rc;
+  e->ops.clear();
+  for (int i = 0; i < e->n; ++i) {
+    const int sb = e->slotbase[i];
+    for (MeshOp o : e->iops[i]) {
+      o.a += sb; o.s1 += sb; o.s2 += sb; o.s3 += sb; o.s4 += sb;
+      e->ops.push_back(o);
+    }
+  }
+  if (changed) *changed = (int)e->ops.size();
+  if (e->ops.empty()) return WX_OK;
+  if ((rc = e->dMeshOps.ensure(e->ops.size()))) return rc;
+  if ((rc = stage_begin(e, al256(e->ops.size() * sizeof(MeshOp))))) return rc;
+  if ((rc = stage_copy(e, e->dMeshOps.ptr, e->ops.data(), e->ops.size() * sizeof(MeshOp)))) return rc;
+  stage_end(e);
+  ProfScope ps(e, KID_MESH);
+  rc = launch_meshops(e->neq, e->dMeshOps.ptr, (int)e->ops.size(), e->cur_u(), e->stream);
+  if (rc) return fail(rc, "meshops launch failed");
+  return WX_OK;
+}
+
+int check_inst(wx_ens *e, int inst) {
+  if (!e) return fail(WX_EARG, "null ensemble");
+  if (inst < 0 || inst >= e->n) return fail(WX_EARG, "instance %d outside [0, %d)", inst, e->n);
+  return WX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wx_ens_create(const wx_params *p, int32_t n, wx_ens **out) {
+  if (!p || !out || n < 1) return fail(WX_EARG, "bad argument");
+  std::string why;
+  for (int i = 0; i < n; ++i) {
+    if (!params_ok(p + i, why)) return fail(WX_EARG, "instance %d: bad parameters: %s", i, why.c_str());
+    if (p[i].neq != p[0].neq) return fail(WX_EARG, "instance %d: neq %d differs from instance 0 (%d)", i, p[i].neq, p[0].neq);
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(WX_EHIP, "no HIP device");
+  wx_ens *e = new wx_ens();
+  e->n = n;
+  e->neq = p[0].neq;
+  e->m.resize(n);
+  e->slotbase.resize(n);
+  e->leafbase.resize(n);
+  e->inst_off.assign((size_t)n + 1, 0);
+  e->rcs.assign(n, 0);
+  e->errs.resize(n);
+  e->iops.resize(n);
+  for (int i = 0; i < n; ++i) {
+    e->m[i].init(p[i]);
+    e->slotbase[i] = (int32_t)e->nslots;
+    e->leafbase[i] = (int32_t)e->nrec;
+    e->nslots += (size_t)p[i].nblocks + 1;
+    e->nrec += (size_t)p[i].nblocks;
+  }
+  if (e->nslots > (size_t)INT32_MAX / 8) { delete e; return fail(WX_EARG, "ensemble too large (%zu slots)", e->nslots); }
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&e->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || e->ncu < 1) e->ncu = 256;
+  int nthreads = 8;
+  if (const char *s = getenv("WX_HOST_THREADS")) nthreads = std::max(1, atoi(s));
+  else if (const char *s2 = getenv("OMP_NUM_THREADS")) nthreads = std::max(1, atoi(s2));
+  nthreads = std::min(nthreads, std::max(1, (int)std::thread::hardware_concurrency()));
+  nthreads = std::min(nthreads, 32);
+  e->pool.reset(new WorkerPool(nthreads));
+  int rc = e->create_stream();
+  const size_t tiles = e->nslots * e->tile();
+  for (int k = 0; k < 2 && !rc; ++k) rc = e->U[k].ensure(tiles);
+  if (!rc) rc = e->dP.ensure(n);
+  if (!rc) rc = e->dS.ensure(n);
+  if (!rc) rc = e->dQ.ensure(1);
+  if (!rc) rc = e->dInstOff.ensure((size_t)n + 1);
+  if (!rc) rc = e->dLeafBase.ensure(n);
+  if (!rc) rc = e->dRegRec.ensure(e->nrec);
+  if (!rc) rc = e->dRec.ensure(e->nrec);
+  if (!rc) rc = e->dBinfo.ensure(e->nslots);
+  if (!rc) rc = e->dRegOpsII.ensure(e->nrec * 8);
+  if (!rc) rc = e->dRegOpsI.ensure(e->nrec * 8);
+  if (!rc) rc = e->dOpsII.ensure(e->nrec * 8);
+  if (!rc) rc = e->dOpsI.ensure(e->nrec * 8);
+  if (!rc) rc = e->dStat.ensure(e->nslots * 2);
+  if (!rc) rc = e->dFlag.ensure(e->nslots);
+  if (rc) { wx_ens_destroy(e); return rc; }
+  if (hipHostMalloc((void **)&e->hS, (size_t)n * sizeof(DevScalars), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+      hipHostMalloc((void **)&e->hFlagQ, e->nrec, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+    wx_ens_destroy(e);
+    return fail(WX_ENOMEM, "pinned mirrors");
+  }
+  if (hipHostGetDevicePointer((void **)&e->hS_dev, e->hS, 0) != hipSuccess ||
+      hipHostGetDevicePointer((void **)&e->hFlagQ_dev, e->hFlagQ, 0) != hipSuccess) {
+    wx_ens_destroy(e);
+    return fail(WX_EHIP, "mapped pointers of the pinned mirrors");
+  }
+  memset(e->hS, 0, (size_t)n * sizeof(DevScalars));
+  memset(e->hFlagQ, 0, e->nrec);
+  for (int k = 0; k < 2; ++k) (void)hipMemsetAsync(e->U[k].ptr, 0, tiles * sizeof(double), e->stream);
+  (void)hipMemsetAsync(e->dS.ptr, 0, (size_t)n * sizeof(DevScalars), e->stream);
+  (void)hipMemsetAsync(e->dQ.ptr, 0, sizeof(QueueHdr), e->stream);
+  (void)hipMemsetAsync(e->dInstOff.ptr, 0, ((size_t)n + 1) * sizeof(int32_t), e->stream);
+  (void)hipMemsetAsync(e->dBinfo.ptr, 0, e->nslots * sizeof(BlockRec), e->stream);
+  (void)hipMemcpyAsync(e->dP.ptr, p, (size_t)n * sizeof(wx_params), hipMemcpyHostToDevice, e->stream);
+  (void)hipMemcpyAsync(e->dLeafBase.ptr, e->leafbase.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, e->stream);
+  if (hipStreamSynchronize(e->stream) != hipSuccess) { wx_ens_destroy(e); return fail(WX_EHIP, "init sync"); }
+  *out = e;
+  return WX_OK;
+}
+
+int wx_ens_destroy(wx_ens *e) {
+  if (!e) return WX_OK;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  drop_graphs(e);
+  for (auto &b : e->U) b.release();
+  e->dP.release(); e->dS.release(); e->dQ.release(); e->dInstOff.release(); e->dLeafBase.release();
+  e->dRegRec.release(); e->dRec.release(); e->dBinfo.release(); e->dRegOpsII.release(); e->dRegOpsI.release();
+  e->dOpsII.release(); e->dOpsI.release();
+  e->dStat.release(); e->dFlag.release(); e->dMeshOps.release(); e->dDirty.release(); e->dSur.release();
+  e->Runtime::destroy();
+  if (e->hS) (void)hipHostFree(e->hS);
+  if (e->hFlagQ) (void)hipHostFree(e->hFlagQ);
+  delete e;
+  return WX_OK;
+}
+
+int wx_ens_basegrid(wx_ens *e, int32_t inst, const double *uroots, int32_t nroots) {
+  int rc = check_inst(e, inst);
+  if (rc) return rc;
+  Mesh &m = e->m[inst];
+  if (!uroots || m.p.nbrootx * m.p.nbrooty != nroots)
+    return fail(WX_EARG, "instance %d: nroots %d != %d x %d", inst, nroots, m.p.nbrootx, m.p.nbrooty);
+  if ((rc = basegrid_tree(&m))) return rc;
+  const size_t bytes = (size_t)(nroots + 1) * e->tile() * sizeof(double);
+  HIPCHK(hipMemcpyAsync(e->cur_u() + (size_t)e->slotbase[inst] * e->tile(), uroots, bytes, hipMemcpyHostToDevice,
+                        e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return WX_OK;
+}
+
+int wx_ens_admesh(wx_ens *e, int32_t *changed) {
+  if (!e) return fail(WX_EARG, "null ensemble");
+  int rc = ensure_stats(e);
+  if (rc) return rc;
+  rc = launch_gather_flags(e->queue(), std::max(1, std::min((e->nleaf_total + 255) / 256, 1024)), e->dFlag.ptr,
+                           e->hFlagQ_dev, e->dS.ptr, e->hS_dev, e->n, e->stream);
+  if (rc) return fail(rc, "flag gather launch failed");
+  int ch = 0;
+  rc = admesh_all(e, &ch);
+  if (changed) *changed = ch;
+  return rc;
+}
+
+int wx_ens_step(wx_ens *e, int32_t nsteps, int32_t do_admesh, double *zone_updates) {
+  if (!e || nsteps < 0) return fail(WX_EARG, "bad argument");
+  for (int i = 0; i < e->n; ++i)
+    if (e->m[i].nbleafs <= 0) return fail(WX_EARG, "instance %d has no mesh (call wx_ens_basegrid first)", i);
+  double z = 0.0;
+  int rc;
+  for (int s = 0; s < nsteps; ++s) {
+    if ((rc = ensure_stats(e))) return rc;
+    if (e->use_graph && !e->prof) {
+      if ((rc = ensure_graphs(e))) return rc;
+      HIPCHK(hipGraphLaunch(e->gexec[e->cur], e->stream));
+    } else {
+      if ((rc = launch_step(e, e->cur, false))) return rc;
+    }
+    e->cur ^= 1;
+    z += (double)e->nleaf_total * NX * NY;
+    if (do_admesh) {
+      if ((rc = admesh_all(e, nullptr))) return rc;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(e->stream));
+  if ((rc = check_scalars(e))) return rc;
+  if (zone_updates) *zone_updates = z;
+  return WX_OK;
+}
+
+int wx_ens_instance(wx_ens *e, int32_t inst, double *time, double *dt, int32_t *nbmax, int32_t *nbleafs) {
+  int rc = check_inst(e, inst);
+  if (rc) return rc;
+  DevScalars s{};
+  HIPCHK(hipMemcpyAsync(&s, e->dS.ptr + inst, sizeof s, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  if (time) *time = s.time;
+  if (dt) *dt = s.dt;
+  if (nbmax) *nbmax = e->m[inst].nbmax;
+  if (nbleafs) *nbleafs = e->m[inst].nbleafs;
+  return WX_OK;
+}
+
+int wx_ens_download(wx_ens *e, int32_t inst, double *u) {
+  int rc = check_inst(e, inst);
+  if (rc) return rc;
+  if (!u) return fail(WX_EARG, "null buffer");
+  const size_t bytes = (size_t)(e->m[inst].nbmax + 1) * e->tile() * sizeof(double);
+  HIPCHK(hipMemcpyAsync(u, e->cur_u() + (size_t)e->slotbase[inst] * e->tile(), bytes, hipMemcpyDeviceToHost,
+                        e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return WX_OK;
+}
+
+int wx_ens_get_tree(wx_ens *e, int32_t inst, int32_t *lp, int32_t *leaf, int32_t *icoord, int32_t ld,
+                    int32_t *nbmax, int32_t *nbleafs) {
+  int rc = check_inst(e, inst);
+  if (rc) return rc;
+  return get_tree(&e->m[inst], lp, leaf, icoord, ld, nbmax, nbleafs);
+}
+
+int wx_ens_set_graph(wx_ens *e, int32_t enable) {
+  if (!e) return fail(WX_EARG, "null ensemble");
+  e->use_graph = enable != 0;
+  if (!e->use_graph) drop_graphs(e);
+  return WX_OK;
+}
+
+int wx_ens_set_grid(wx_ens *e, int32_t kernel, int32_t grid) {
+  if (!e) return fail(WX_EARG, "null ensemble");
+  if (kernel < 0 || kernel > 2 || grid < 0) return fail(WX_EARG, "kernel %d / grid %d", kernel, grid);
+  e->grid[kernel] = grid;
+  drop_graphs(e);
+  return WX_OK;
+}
+
+// Launch-shape autotuner: times each stage kernel over `reps` launches at
+// several persistent grid sizes (1, 2, 4, 8 workgroups per CU, and one
+// workgroup per queue entry) on the current meshes and keeps the fastest.
+// Stage A and the fill are idempotent given Uⁿ; stage B updates in place,
+// so it runs on a scratch copy of Uⁿ.  The statistics the fill's epilogue
+// overwrites are recomputed afterwards, so tuning leaves the results of
+// later steps unchanged.
+int wx_ens_tune(wx_ens *e, int32_t reps, int32_t *grids, double *us) {
+  if (!e || reps < 1) return fail(WX_EARG, "bad argument");
+  int rc = ensure_stats(e);
+  if (rc) return rc;
+  if (e->nleaf_total == 0) return WX_OK;
+  HIPCHK(hipStreamSynchronize(e->stream));
+  std::vector<DevScalars> saved(e->n);
+  HIPCHK(hipMemcpy(saved.data(), e->dS.ptr, (size_t)e->n * sizeof(DevScalars), hipMemcpyDeviceToHost));
+  const size_t tiles = e->nslots * e->tile();
+  DevBuf<double> scratch;
+  if ((rc = scratch.ensure(tiles))) return rc;
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  const QueueRef q = e->queue();
+  std::vector<int> cands;
+  for (int w : {1, 2, 4, 8, 16})
+    if (e->ncu * w < e->nleaf_total) cands.push_back(e->ncu * w);
+  cands.push_back(e->nleaf_total);
+  for (int k = 0; k < 3 && !rc; ++k) {
+    double best = 1e30;
+    int bestg = e->nleaf_total;
+    for (int g : cands) {
+      float tot = 0.f;
+      for (int r = 0; r <= reps && !rc; ++r) {   // r = 0 warms up
+        if (k == 1 && hipMemcpyAsync(scratch.ptr, e->cur_u(), tiles * sizeof(double), hipMemcpyDeviceToDevice,
+                                     e->stream) != hipSuccess) rc = fail(WX_EHIP, "tune copy");
+        (void)hipEventRecord(a, e->stream);
+        if (k == 0) rc = launch_stageA(e->dP.ptr, e->neq, q, g, e->cur_u(), e->other_u(), e->dS.ptr, e->stream);
+        else if (k == 1) rc = launch_stageB(e->dP.ptr, e->neq, q, g, e->dOpsII.ptr, e->dBinfo.ptr, scratch.ptr,
+                                            e->other_u(), e->dS.ptr, e->stream);
+        else rc = launch_fillI_cool(e->dP.ptr, e->neq, q, g, e->dOpsI.ptr, e->dBinfo.ptr, e->cur_u(), scratch.ptr,
+                                    e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, e->stream);
+        (void)hipEventRecord(b, e->stream);
+        (void)hipEventSynchronize(b);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        if (r > 0) tot += ms;
+      }
+      const double t = 1e3 * tot / reps;
+      if (t < best) { best = t; bestg = g; }
+    }
+    e->grid[k] = bestg == e->nleaf_total ? 0 : bestg;
+    if (grids) grids[k] = bestg;
+    if (us) us[k] = best;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  HIPCHK(hipStreamSynchronize(e->stream));
+  scratch.release();
+  HIPCHK(hipMemcpy(e->dS.ptr, saved.data(), (size_t)e->n * sizeof(DevScalars), hipMemcpyHostToDevice));
+  for (auto &m : e->m) m.stats_all_stale = true;
+  drop_graphs(e);
+  return rc;
+}
+
+int wx_ens_set_surrogate(wx_ens *e, const wx_sur_model *m) {
+  if (!e) return fail(WX_EARG, "null ensemble");
+  drop_graphs(e);
+  if (!m) {
+    e->use_sur = false;
+    return WX_OK;
+  }
+  if (e->neq < 5) return fail(WX_EARG, "the cooling surrogate replaces coolingh (neq >= 5)");
+  int rc = e->dSur.ensure(1);
+  if (rc) return rc;
+  HIPCHK(hipMemsetAsync(e->dSur.ptr, 0, sizeof(SurDev), e->stream));
+  // the loss table depends only on the routine's constants, the same for
+  // every instance (instance 0's parameters fill it)
+  if ((rc = sur_upload(e->dSur.ptr, m, e->dP.ptr, e->stream))) return rc;
+  HIPCHK(hipStreamSynchronize(e->stream));
+  e->use_sur = true;
+  return WX_OK;
+}
+
+int wx_ens_sur_counters(wx_ens *e, int32_t reset, int64_t *cells, int64_t *fallbacks) {
+  if (!e) return fail(WX_EARG, "null ensemble");
+  if (!e->dSur.ptr) {
+    if (cells) *cells = 0;
+    if (fallbacks) *fallbacks = 0;
+    return WX_OK;
+  }
+  return sur_counters(e->dSur.ptr, e->stream, reset, cells, fallbacks);
+}
+
+int wx_ens_profile(wx_ens *e, int32_t enable, double *ms, int64_t *counts, int32_t n) {
+  if (!e) return fail(WX_EARG, "null ensemble");
+  return prof_report(e, enable, ms, counts, n);
+}
+
+int wx_ens_info(wx_ens *e, int32_t *ninst, int32_t *nleaf_total, int32_t *grids) {
+  if (!e) return fail(WX_EARG, "null ensemble");
+  if (ninst) *ninst = e->n;
+  if (nleaf_total) *nleaf_total = e->nleaf_total;
+  if (grids) for (int k = 0; k < 3; ++k) grids[k] = e->grid[k];
+  return WX_OK;
+}
+
+}  // extern "C"

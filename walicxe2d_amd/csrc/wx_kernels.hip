@@ -17,6 +17,7 @@
 //                 flag of the block (fused timestep/criteria epilogue).
 #include <hip/hip_runtime.h>
 #include "wx_device.h"
+#include "wx_mlp.h"
 #include "wx_physics.h"
 
 namespace wx {
@@ -197,14 +198,20 @@ __device__ __forceinline__ void cyl_source(const wx_params *__restrict__ p, cons
 // ---------------------------------------------------------------------
 // Stage A: src/tstep.f90:22-46 (hllcfluxes choice 1 + step with dt/2).
 template <int NEQ>
-__global__ __launch_bounds__(256) void k_stageA(const wx_params *__restrict__ p,
-                                                const LeafRec *__restrict__ leaves,
+__global__ __launch_bounds__(256) void k_stageA(const wx_params *__restrict__ P, QueueRef q,
                                                 const double *__restrict__ U, double *__restrict__ Q,
-                                                DevScalars *__restrict__ S) {
+                                                DevScalars *__restrict__ Sa) {
   __shared__ double prim[NEQ * NC];
   __shared__ double F[NEQ * 272];
   __shared__ double G[NEQ * 272];
-  const LeafRec L = leaves[blockIdx.x];
+  // the first entry's record is read together with the queue length
+  LeafRec Lnext = q.rec[min((int)blockIdx.x, q.cap - 1)];
+  const int nleaf = q.hdr->nleaf;
+  for (int l = blockIdx.x; l < nleaf; l += gridDim.x) {
+  const LeafRec L = Lnext;
+  if (l + (int)gridDim.x < nleaf) Lnext = q.rec[l + gridDim.x];
+  const wx_params *__restrict__ p = P + L.inst;
+  DevScalars *__restrict__ S = Sa + L.inst;
   const double *ub = U + (size_t)L.id * NEQ * NC;
   // rows j = 0..17 are one contiguous range [NP, 19*NP) of every variable
   for (int c = threadIdx.x; c < 18 * NP; c += blockDim.x) {
@@ -233,22 +240,29 @@ __global__ __launch_bounds__(256) void k_stageA(const wx_params *__restrict__ p,
     qb[k * NC + IDX(i, j)] = v;
   }
   if (err) atomicOr(&S->err, WX_EHLLC);
+  __syncthreads();   // the next queue entry reuses the LDS tiles
+  }
 }
 
 // ---------------------------------------------------------------------
 // Stage B: src/tstep.f90:53-88 with boundaryII pulled from the neighbours.
 template <int NEQ>
-__global__ __launch_bounds__(256) void k_stageB(const wx_params *__restrict__ p,
-                                                const LeafRec *__restrict__ leaves,
+__global__ __launch_bounds__(256) void k_stageB(const wx_params *__restrict__ P, QueueRef q,
                                                 const int2 *__restrict__ ops,
                                                 const BlockRec *__restrict__ binfo,
                                                 double *__restrict__ U, const double *__restrict__ Q,
-                                                DevScalars *__restrict__ S) {
+                                                DevScalars *__restrict__ Sa) {
   __shared__ double prim[NEQ * NC];
   __shared__ double F[NEQ * 272];
   __shared__ double G[NEQ * 272];
-  const int l = blockIdx.x;
-  const LeafRec L = leaves[l];
+  LeafRec Lnext = q.rec[min((int)blockIdx.x, q.cap - 1)];
+  const int nleaf = q.hdr->nleaf;
+  for (int l = blockIdx.x; l < nleaf; l += gridDim.x) {
+  const int r = l;
+  const LeafRec L = Lnext;
+  if (l + (int)gridDim.x < nleaf) Lnext = q.rec[l + gridDim.x];
+  const wx_params *__restrict__ p = P + L.inst;
+  DevScalars *__restrict__ S = Sa + L.inst;
   const int b = L.id;
   const double time = S->time;
   const bool imp = (L.flags & 1) != 0;
@@ -273,7 +287,7 @@ __global__ __launch_bounds__(256) void k_stageB(const wx_params *__restrict__ p,
   const bool ghost = tid < 128;
   if (ghost) {
     const int side = tid >> 5, layer = ((tid >> 4) & 1) + 1, pos = (tid & 15) + 1;
-    const int2 op = ops[l * 8 + side * 2 + ((pos - 1) >> 3)];
+    const int2 op = ops[r * 8 + side * 2 + ((pos - 1) >> 3)];
     ghost_gather<NEQ>(p, Q, binfo, op, b, L.level, L.icx, L.icy, side, layer, pos, vjet, gh);
     ghost_coords(side, layer, pos, gi, gj);
     double q[NEQ], T;
@@ -311,6 +325,8 @@ __global__ __launch_bounds__(256) void k_stageB(const wx_params *__restrict__ p,
     for (int k = 0; k < NEQ; ++k) ub[k * NC + IDX(gi, gj)] = gh[k];
   }
   if (err) atomicOr(&S->err, WX_EHLLC);
+  __syncthreads();   // the next queue entry reuses the LDS tiles
+  }
 }
 
 // ---------------------------------------------------------------------
@@ -372,19 +388,112 @@ __device__ void block_stats(const wx_params *__restrict__ p, const double *prim,
 
 // ---------------------------------------------------------------------
 // boundaryI + cooling (+ CFL / criteria epilogue): src/tstep.f90:92-122.
+// Cooling surrogate on the 18x18 cells of a tile (coolingh instances only):
+// features per cell, the network per wave on groups of 16 cells (matrix
+// cores), then per cell either the network's update or -- outside the
+// training hull or when the energy residual exceeds the tolerance -- the
+// exact routine.  The network's temperature output is a correction to the
+// routine's relaxation law T1 = 10 + (T - 10) exp(-ce dt)
+// (src/coolingh.f90:249-250) evaluated with the tabulated loss; the energy
+// residual is the change of thermal energy (2 - y1) T1 that correction
+// makes, relative to the old thermal energy (2 - y0) T.
 template <int NEQ>
-__global__ __launch_bounds__(256) void k_fillI_cool(const wx_params *__restrict__ p,
-                                                    const LeafRec *__restrict__ leaves,
+__device__ void cool_surrogate(const wx_params *__restrict__ p, SurDev *__restrict__ M, double dts, double *tile) {
+  __shared__ float sx[324 * 4];
+  __shared__ float sy[324 * 2];
+  __shared__ float scedt[324];
+  __shared__ uint8_t sinh[324];
+  __shared__ unsigned int nfb;
+  const int tid = threadIdx.x;
+  if (tid == 0) nfb = 0u;
+  for (int c = tid; c < 324; c += blockDim.x) {
+    const int i = c % 18, j = c / 18;
+    double uu[NEQ], q[NEQ], T;
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) uu[k] = tile[k * NC + IDX(i, j)];
+    uprim<NEQ>(p, q, uu, T);
+    float f[4], cedt;
+    sinh[c] = sur_features(M, uu[0], T, uu[4] / uu[0], dts, f, cedt) ? 1 : 0;
+    scedt[c] = cedt;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sx[4 * c + k] = (f[k] - M->in_mu[k]) / M->in_sig[k];
+  }
+  __syncthreads();
+  {
+    const int l = tid & 63, r = l & 15, qq = l >> 4;
+    for (int g = tid >> 6; g < 21; g += 4) {   // 21 groups of 16 cover the 324 cells
+      const int c = 16 * g + r;
+      bf16x4 xb = {0, 0, 0, 0};
+      if (qq == 0 && c < 324) xb = pack4(sx[4 * c], sx[4 * c + 1], sx[4 * c + 2], sx[4 * c + 3]);
+      const f32x4 y = mlp_fwd16(M->w1, M->w2, M->w3, M->w4, M->theta, xb);
+      if (qq == 0 && c < 324) {
+        sy[2 * c] = y[0];
+        sy[2 * c + 1] = y[1];
+      }
+    }
+  }
+  __syncthreads();
+  unsigned int mine = 0;
+  for (int c = tid; c < 324; c += blockDim.x) {
+    const int i = c % 18, j = c / 18;
+    double uu[NEQ], q[NEQ], T;
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) uu[k] = tile[k * NC + IDX(i, j)];
+    uprim<NEQ>(p, q, uu, T);
+    const double y0 = uu[4] / uu[0];
+    const float corr = sy[2 * c] * M->out_sig[0] + M->out_mu[0];
+    const float dx = sy[2 * c + 1] * M->out_sig[1] + M->out_mu[1];
+    double y1 = y0 - (double)dx;
+    y1 = dmin(y1, 1.);
+    y1 = dmax(y1, 0.0001);
+    // the relaxation law with the tabulated loss, then the network's
+    // correction log10(T1 / T1_table)
+    double tt = 10. + (T - 10.) * (double)__expf(-scedt[c]);
+    tt = dmax(tt, 0.1 * T);
+    tt = dmin(tt, 10. * T);
+    tt = dmax(tt, 10.);
+    double t1 = tt * (double)exp10f(corr);
+    t1 = dmax(t1, 0.1 * T);
+    t1 = dmin(t1, 10. * T);
+    t1 = dmax(t1, 10.);
+    const double resid = (2. - y1) * fabs(t1 - tt) / ((2. - y0) * T);
+    if (!sinh[c] || !(resid <= (double)M->tol)) {
+      atomic_h<NEQ>(p, dts, uu);
+      ++mine;
+    } else {
+      uu[4] = y1 * uu[0];
+      uu[3] = p->cv * (2. * uu[0] - uu[4]) * t1 / p->Tempsc + 0.5 * q[0] * (q[1] * q[1] + q[2] * q[2]);
+    }
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) tile[k * NC + IDX(i, j)] = uu[k];
+  }
+  if (mine) atomicAdd(&nfb, mine);
+  __syncthreads();
+  if (tid == 0) {
+    atomicAdd(&M->fallbacks, (unsigned long long)nfb);
+    atomicAdd(&M->cells, 324ull);
+  }
+}
+
+template <int NEQ, bool SUR>
+__global__ __launch_bounds__(256) void k_fillI_cool(const wx_params *__restrict__ P, QueueRef q,
                                                     const int2 *__restrict__ ops,
                                                     const BlockRec *__restrict__ binfo,
                                                     const double *__restrict__ Uin,
                                                     double *__restrict__ Uout,
                                                     double *__restrict__ stat,
                                                     int8_t *__restrict__ flag,
-                                                    DevScalars *__restrict__ S) {
+                                                    DevScalars *__restrict__ Sa,
+                                                    SurDev *__restrict__ M) {
   __shared__ double tile[NEQ * NC];
-  const int l = blockIdx.x;
-  const LeafRec L = leaves[l];
+  LeafRec Lnext = q.rec[min((int)blockIdx.x, q.cap - 1)];
+  const int nleaf = q.hdr->nleaf;
+  for (int l = blockIdx.x; l < nleaf; l += gridDim.x) {
+  const int r = l;
+  const LeafRec L = Lnext;
+  if (l + (int)gridDim.x < nleaf) Lnext = q.rec[l + gridDim.x];
+  const wx_params *__restrict__ p = P + L.inst;
+  DevScalars *__restrict__ S = Sa + L.inst;
   const int b = L.id;
   const int tid = threadIdx.x;
   const double time = S->time;
@@ -407,7 +516,7 @@ __global__ __launch_bounds__(256) void k_fillI_cool(const wx_params *__restrict_
   // ghost layer 1 from the list (64 cells)
   if (tid < 64) {
     const int side = tid >> 4, pos = (tid & 15) + 1;
-    const int2 op = ops[l * 8 + side * 2 + ((pos - 1) >> 3)];
+    const int2 op = ops[r * 8 + side * 2 + ((pos - 1) >> 3)];
     if ((op.y & 15) != GK_NONE) {
       double g[NEQ];
       ghost_gather<NEQ>(p, Uin, binfo, op, b, L.level, L.icx, L.icy, side, 1, pos, vjet, g);
@@ -420,7 +529,15 @@ __global__ __launch_bounds__(256) void k_fillI_cool(const wx_params *__restrict_
   __syncthreads();
   // cooling over cells 0..n+1 (src/coolingh.f90:19-20, src/coolingdmc.f90:23-24)
   const int mode = p->cooling;
-  if (mode != 0) {
+  bool done = false;
+  if constexpr (SUR && NEQ >= 5) {
+    if (mode == 1) {
+      cool_surrogate<NEQ>(p, M, S->dt * p->tsc, tile);
+      __syncthreads();
+      done = true;
+    }
+  }
+  if (!done && mode != 0) {
     const double dts = S->dt * p->tsc;
     for (int c = tid; c < 18 * 18; c += blockDim.x) {
       const int i = c % 18, j = c / 18;
@@ -449,11 +566,13 @@ __global__ __launch_bounds__(256) void k_fillI_cool(const wx_params *__restrict_
   }
   __syncthreads();
   block_stats<NEQ>(p, tile, L, stat, flag);
+  __syncthreads();   // the next queue entry reuses the LDS tile
+  }
 }
 
 // Statistics only (after upload or a mesh change).
 template <int NEQ>
-__global__ __launch_bounds__(256) void k_blockstat(const wx_params *__restrict__ p,
+__global__ __launch_bounds__(256) void k_blockstat(const wx_params *__restrict__ P,
                                                    const LeafRec *__restrict__ leaves,
                                                    const double *__restrict__ U,
                                                    double *__restrict__ stat,
@@ -461,6 +580,7 @@ __global__ __launch_bounds__(256) void k_blockstat(const wx_params *__restrict__
   __shared__ double tile[NEQ * NC];
   const int l = blockIdx.x;
   const LeafRec L = leaves[l];
+  const wx_params *__restrict__ p = P + L.inst;
   const double *ub = U + (size_t)L.id * NEQ * NC;
   for (int c = threadIdx.x; c < 18 * 18; c += blockDim.x) {
     const int i = c % 18, j = c / 18;
@@ -476,15 +596,21 @@ __global__ __launch_bounds__(256) void k_blockstat(const wx_params *__restrict__
 }
 
 // dt = min(cfl*dtp, 0.4*dtcool*Psc/tsc) (src/timestep.f90:50); optionally
-// time += dt (src/main.f90:88).  One workgroup.
-__global__ __launch_bounds__(1024) void k_dtreduce(const wx_params *__restrict__ p,
-                                                   const LeafRec *__restrict__ leaves,
-                                                   const double *__restrict__ stat, int nleaf,
-                                                   DevScalars *__restrict__ S, int advance) {
+// time += dt (src/main.f90:88).  Workgroup i reduces instance i's records.
+__global__ __launch_bounds__(1024) void k_dtreduce(const wx_params *__restrict__ P,
+                                                   const LeafRec *__restrict__ rec,
+                                                   const double *__restrict__ stat,
+                                                   const int32_t *__restrict__ inst_off,
+                                                   DevScalars *__restrict__ Sa, int advance) {
   __shared__ double r0[16], r1[16];
+  const int inst = blockIdx.x;
+  const wx_params *__restrict__ p = P + inst;
+  DevScalars *__restrict__ S = Sa + inst;
+  const int n = inst_off[inst + 1] - inst_off[inst];
+  const LeafRec *__restrict__ mine = rec + inst_off[inst];
   double a = 1.e30, c = 1e30;
-  for (int l = threadIdx.x; l < nleaf; l += blockDim.x) {
-    const int id = leaves[l].id;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const int id = mine[k].id;
     a = dmin(a, stat[2 * id]);
     c = dmin(c, stat[2 * id + 1]);
   }
@@ -503,6 +629,46 @@ __global__ __launch_bounds__(1024) void k_dtreduce(const wx_params *__restrict__
     S->dt = dt;
     if (advance) S->time = S->time + dt;
   }
+}
+
+// Compacted queue entry l = record leafbase[i] + l - inst_off[i] of the
+// per-instance regions, i the instance with inst_off[i] <= l < inst_off[i+1]
+// (binary search); copies the record and its 8 + 8 ghost ops.
+__global__ __launch_bounds__(256) void k_build_queue(const QueueHdr *__restrict__ hdr,
+                                                     const int32_t *__restrict__ inst_off,
+                                                     const int32_t *__restrict__ leafbase,
+                                                     const LeafRec *__restrict__ reg_rec,
+                                                     const int2 *__restrict__ reg_ops2,
+                                                     const int2 *__restrict__ reg_ops1,
+                                                     LeafRec *__restrict__ rec, int2 *__restrict__ ops2,
+                                                     int2 *__restrict__ ops1) {
+  const int n = hdr->nleaf, ni = hdr->ninst;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < 8 * n; t += gridDim.x * blockDim.x) {
+    const int l = t >> 3, k = t & 7;
+    int lo = 0, hi = ni - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (inst_off[mid] <= l) lo = mid;
+      else hi = mid - 1;
+    }
+    const size_t src = (size_t)leafbase[lo] + (size_t)(l - inst_off[lo]);
+    if (k == 0) rec[l] = reg_rec[src];
+    ops2[(size_t)l * 8 + k] = reg_ops2[src * 8 + k];
+    ops1[(size_t)l * 8 + k] = reg_ops1[src * 8 + k];
+  }
+}
+
+// Refinement flags in queue order for the host's admesh, and the instances'
+// scalars; both outputs may be pinned host memory (mapped), so that the host
+// reads them after the stream synchronises with no copy launched.
+__global__ __launch_bounds__(256) void k_gather_flags(QueueRef q, const int8_t *__restrict__ flag,
+                                                      int8_t *__restrict__ flagq, const DevScalars *__restrict__ S,
+                                                      DevScalars *__restrict__ Sout, int ninst) {
+  const int n = q.hdr->nleaf;
+  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < n; l += gridDim.x * blockDim.x)
+    flagq[l] = flag[q.rec[l].id];
+  if (Sout && blockIdx.x == 0)
+    for (int i = threadIdx.x; i < ninst; i += blockDim.x) Sout[i] = S[i];
 }
 
 // ---------------------------------------------------------------------
@@ -616,55 +782,83 @@ __global__ void k_calcprim(const wx_params *__restrict__ p, int cap, const doubl
 
 static inline int hip_status() { return hipGetLastError() == hipSuccess ? WX_OK : WX_EHIP; }
 
-int launch_stageA(const wx_params *dP, int neq, const LeafRec *leaves, int nleaf, const double *U,
-                  double *Q, DevScalars *S, void *stream) {
+int launch_build_queue(const QueueHdr *hdr, const int32_t *inst_off, const int32_t *leafbase,
+                       const LeafRec *reg_rec, const int2 *reg_ops2, const int2 *reg_ops1, LeafRec *rec,
+                       int2 *ops2, int2 *ops1, int grid, hipStream_t stream) {
+  if (grid <= 0) return WX_OK;
+  hipLaunchKernelGGL(k_build_queue, dim3(grid), dim3(256), 0, stream, hdr, inst_off, leafbase, reg_rec, reg_ops2,
+                     reg_ops1, rec, ops2, ops1);
+  return hip_status();
+}
+
+int launch_stageA(const wx_params *P, int neq, QueueRef q, int grid, const double *U, double *Q, DevScalars *S,
+                  hipStream_t stream) {
+  if (grid <= 0) return WX_OK;
+  WX_DISPATCH(neq, hipLaunchKernelGGL(k_stageA<N>, dim3(grid), dim3(256), 0, stream, P, q, U, Q, S));
+  return hip_status();
+}
+
+int launch_stageB(const wx_params *P, int neq, QueueRef q, int grid, const int2 *ops, const BlockRec *binfo,
+                  double *U, const double *Q, DevScalars *S, hipStream_t stream) {
+  if (grid <= 0) return WX_OK;
+  WX_DISPATCH(neq, hipLaunchKernelGGL(k_stageB<N>, dim3(grid), dim3(256), 0, stream, P, q, ops, binfo, U, Q, S));
+  return hip_status();
+}
+
+int launch_fillI_cool(const wx_params *P, int neq, QueueRef q, int grid, const int2 *ops, const BlockRec *binfo,
+                      const double *Uin, double *Uout, double *stat, int8_t *flag, DevScalars *S,
+                      hipStream_t stream) {
+  if (grid <= 0) return WX_OK;
+  WX_DISPATCH(neq, hipLaunchKernelGGL((k_fillI_cool<N, false>), dim3(grid), dim3(256), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, nullptr));
+  return hip_status();
+}
+
+int launch_fillI_cool_sur(const wx_params *P, int neq, QueueRef q, int grid, const int2 *ops, const BlockRec *binfo,
+                          const double *Uin, double *Uout, double *stat, int8_t *flag, DevScalars *S, SurDev *M,
+                          hipStream_t stream) {
+  if (grid <= 0) return WX_OK;
+  switch (neq) {
+    case 5: hipLaunchKernelGGL((k_fillI_cool<5, true>), dim3(grid), dim3(256), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, M); break;
+    case 6: hipLaunchKernelGGL((k_fillI_cool<6, true>), dim3(grid), dim3(256), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, M); break;
+    default: return WX_EARG;   // coolingh needs u(5) = n_HI
+  }
+  return hip_status();
+}
+
+int launch_blockstat(const wx_params *P, int neq, const LeafRec *leaves, int nleaf, const double *U,
+                     double *stat, int8_t *flag, hipStream_t stream) {
   if (nleaf <= 0) return WX_OK;
-  WX_DISPATCH(neq, hipLaunchKernelGGL(k_stageA<N>, dim3(nleaf), dim3(256), 0, (hipStream_t)stream, dP, leaves, U, Q, S));
+  WX_DISPATCH(neq, hipLaunchKernelGGL(k_blockstat<N>, dim3(nleaf), dim3(256), 0, stream, P, leaves, U, stat, flag));
   return hip_status();
 }
 
-int launch_stageB(const wx_params *dP, int neq, const LeafRec *leaves, int nleaf, const int2 *ops,
-                  const BlockRec *binfo, double *U, const double *Q, DevScalars *S, void *stream) {
-  if (nleaf <= 0) return WX_OK;
-  WX_DISPATCH(neq, hipLaunchKernelGGL(k_stageB<N>, dim3(nleaf), dim3(256), 0, (hipStream_t)stream, dP, leaves, ops, binfo, U, Q, S));
+int launch_dtreduce(const wx_params *P, const LeafRec *rec, const double *stat, const int32_t *inst_off,
+                    int ninst, DevScalars *S, int advance, hipStream_t stream) {
+  if (ninst <= 0) return WX_OK;
+  hipLaunchKernelGGL(k_dtreduce, dim3(ninst), dim3(1024), 0, stream, P, rec, stat, inst_off, S, advance);
   return hip_status();
 }
 
-int launch_fillI_cool(const wx_params *dP, int neq, const LeafRec *leaves, int nleaf, const int2 *ops,
-                      const BlockRec *binfo, const double *Uin, double *Uout, double *stat,
-                      int8_t *flag, DevScalars *S, void *stream) {
-  if (nleaf <= 0) return WX_OK;
-  WX_DISPATCH(neq, hipLaunchKernelGGL(k_fillI_cool<N>, dim3(nleaf), dim3(256), 0, (hipStream_t)stream, dP, leaves, ops, binfo, Uin, Uout, stat, flag, S));
+int launch_gather_flags(QueueRef q, int grid, const int8_t *flag, int8_t *flagq, const DevScalars *S,
+                        DevScalars *Sout, int ninst, hipStream_t stream) {
+  if (grid <= 0) return WX_OK;
+  hipLaunchKernelGGL(k_gather_flags, dim3(grid), dim3(256), 0, stream, q, flag, flagq, S, Sout, ninst);
   return hip_status();
 }
 
-int launch_blockstat(const wx_params *dP, int neq, const LeafRec *leaves, int nleaf, const double *U,
-                     double *stat, int8_t *flag, void *stream) {
-  if (nleaf <= 0) return WX_OK;
-  WX_DISPATCH(neq, hipLaunchKernelGGL(k_blockstat<N>, dim3(nleaf), dim3(256), 0, (hipStream_t)stream, dP, leaves, U, stat, flag));
-  return hip_status();
-}
-
-int launch_dtreduce(const wx_params *dP, const LeafRec *leaves, const double *stat, int nleaf, DevScalars *S,
-                    int advance, void *stream) {
-  hipLaunchKernelGGL(k_dtreduce, dim3(1), dim3(1024), 0, (hipStream_t)stream, dP, leaves, stat, nleaf, S, advance);
-  return hip_status();
-}
-
-int launch_meshops(const wx_params *dP, int neq, const MeshOp *ops, int nops, double *U, void *stream) {
-  (void)dP;
+int launch_meshops(int neq, const MeshOp *ops, int nops, double *U, hipStream_t stream) {
   if (nops <= 0) return WX_OK;
-  WX_DISPATCH(neq, hipLaunchKernelGGL(k_meshops<N>, dim3(nops), dim3(256), 0, (hipStream_t)stream, ops, U));
+  WX_DISPATCH(neq, hipLaunchKernelGGL(k_meshops<N>, dim3(nops), dim3(256), 0, stream, ops, U));
   return hip_status();
 }
 
-int launch_transpose(int neq, int cap, const double *src, double *dst, int to_fortran, int ld, void *stream) {
-  WX_DISPATCH(neq, hipLaunchKernelGGL(k_transpose<N>, dim3(1024), dim3(256), 0, (hipStream_t)stream, cap, src, dst, to_fortran, ld));
+int launch_transpose(int neq, int cap, const double *src, double *dst, int to_fortran, int ld, hipStream_t stream) {
+  WX_DISPATCH(neq, hipLaunchKernelGGL(k_transpose<N>, dim3(1024), dim3(256), 0, stream, cap, src, dst, to_fortran, ld));
   return hip_status();
 }
 
-int launch_calcprim(const wx_params *dP, int neq, int cap, const double *U, double *prim, void *stream) {
-  WX_DISPATCH(neq, hipLaunchKernelGGL(k_calcprim<N>, dim3(1024), dim3(256), 0, (hipStream_t)stream, dP, cap, U, prim));
+int launch_calcprim(const wx_params *P, int neq, int cap, const double *U, double *prim, hipStream_t stream) {
+  WX_DISPATCH(neq, hipLaunchKernelGGL(k_calcprim<N>, dim3(1024), dim3(256), 0, stream, P, cap, U, prim));
   return hip_status();
 }
 

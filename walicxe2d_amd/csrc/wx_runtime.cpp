@@ -1,0 +1,204 @@
+// wx_runtime.cpp -- stream, pinned upload staging and event timing shared
+// by the single-mesh context and the ensemble.
+#include <cstring>
+
+#include "wx_runtime.h"
+
+namespace wx {
+
+int Runtime::create_stream() {
+  if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
+    stream = nullptr;
+    return fail(WX_EHIP, "hipStreamCreate failed");
+  }
+  return WX_OK;
+}
+
+void Runtime::destroy() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (auto &st : stage) {
+    if (st.ev) (void)hipEventDestroy(st.ev);
+    if (st.buf) (void)hipHostFree(st.buf);
+    st = Stage{};
+  }
+  prof_collect(this);
+  for (auto e : evpool) (void)hipEventDestroy(e);
+  evpool.clear();
+  if (stream) (void)hipStreamDestroy(stream);
+  stream = nullptr;
+}
+
+// Opens the next staging buffer for `total` bytes (waits only if that
+// buffer's previous copies have not finished).
+int stage_begin(Runtime *r, size_t total) {
+  r->stage_cur ^= 1;
+  Runtime::Stage &st = r->stage[r->stage_cur];
+  if (st.pending) {
+    (void)hipEventSynchronize(st.ev);
+    st.pending = false;
+  }
+  if (!st.ev && hipEventCreateWithFlags(&st.ev, hipEventDisableTiming) != hipSuccess)
+    return fail(WX_EHIP, "hipEventCreate failed");
+  if (st.cap < total) {
+    if (st.buf) (void)hipHostFree(st.buf);
+    st.buf = nullptr;
+    st.cap = 0;
+    size_t want = total + total / 2 + 4096;
+    if (hipHostMalloc((void **)&st.buf, want) != hipSuccess) return fail(WX_ENOMEM, "pinned staging %zu bytes", want);
+    st.cap = want;
+  }
+  st.used = 0;
+  return WX_OK;
+}
+
+int stage_copy(Runtime *r, void *dst, const void *src, size_t bytes) {
+  if (!bytes) return WX_OK;
+  Runtime::Stage &st = r->stage[r->stage_cur];
+  if (st.used + bytes > st.cap) return fail(WX_EARG, "staging overflow (%zu + %zu > %zu)", st.used, bytes, st.cap);
+  char *p = st.buf + st.used;
+  memcpy(p, src, bytes);
+  st.used += al256(bytes);
+  HIPCHK(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, r->stream));
+  return WX_OK;
+}
+
+char *stage_take(Runtime *r, size_t bytes) {
+  Runtime::Stage &st = r->stage[r->stage_cur];
+  if (st.used + bytes > st.cap) return nullptr;
+  char *p = st.buf + st.used;
+  st.used += al256(bytes);
+  return p;
+}
+
+int stage_push(Runtime *r, void *dst, const char *host, size_t bytes) {
+  if (!bytes) return WX_OK;
+  HIPCHK(hipMemcpyAsync(dst, host, bytes, hipMemcpyHostToDevice, r->stream));
+  return WX_OK;
+}
+
+void stage_end(Runtime *r) {
+  Runtime::Stage &st = r->stage[r->stage_cur];
+  (void)hipEventRecord(st.ev, r->stream);
+  st.pending = true;
+}
+
+static hipEvent_t take_event(Runtime *r) {
+  if (!r->evpool.empty()) {
+    hipEvent_t e = r->evpool.back();
+    r->evpool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+ProfScope::ProfScope(Runtime *r_, int kid_) : r(r_), kid(kid_) {
+  if (r->prof && kid >= 0) {
+    a = take_event(r);
+    (void)hipEventRecord(a, r->stream);
+  }
+}
+
+ProfScope::~ProfScope() {
+  if (r->prof && a) {
+    hipEvent_t b = take_event(r);
+    (void)hipEventRecord(b, r->stream);
+    r->marks.push_back(Runtime::Mark{kid, a, b});
+  }
+}
+
+HostTimer::~HostTimer() {
+  if (!r->prof) return;
+  r->prof_ms[slot] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  r->prof_n[slot] += 1;
+}
+
+void prof_collect(Runtime *r) {
+  if (r->marks.empty()) return;
+  (void)hipStreamSynchronize(r->stream);
+  for (auto &m : r->marks) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, m.a, m.b) == hipSuccess) {
+      r->prof_ms[m.kid] += ms;
+      r->prof_n[m.kid] += 1;
+    }
+    r->evpool.push_back(m.a);
+    r->evpool.push_back(m.b);
+  }
+  r->marks.clear();
+}
+
+int prof_report(Runtime *r, int32_t enable, double *ms, int64_t *counts, int32_t n) {
+  prof_collect(r);
+  for (int k = 0; k < NPROF && k < n; ++k) {
+    if (ms) ms[k] = r->prof_ms[k];
+    if (counts) counts[k] = r->prof_n[k];
+  }
+  if (enable >= 0) {
+    r->prof = enable != 0;
+    for (int k = 0; k < NPROF; ++k) {
+      r->prof_ms[k] = 0;
+      r->prof_n[k] = 0;
+    }
+  }
+  return WX_OK;
+}
+
+WorkerPool::WorkerPool(int nthreads) {
+  for (int k = 1; k < nthreads; ++k) threads_.emplace_back([this] { loop(); });
+}
+
+WorkerPool::~WorkerPool() {
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    stop_ = true;
+  }
+  cv_.notify_all();
+  for (auto &t : threads_) t.join();
+}
+
+void WorkerPool::loop() {
+  uint64_t seen = 0;
+  for (;;) {
+    std::unique_lock<std::mutex> lk(mu_);
+    cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+    if (stop_) return;
+    seen = gen_;
+    ++active_;
+    while (next_ < n_) {
+      const int i = next_++;
+      lk.unlock();
+      (*fn_)(i);
+      lk.lock();
+    }
+    if (--active_ == 0) done_cv_.notify_all();
+  }
+}
+
+void WorkerPool::run(int n, const std::function<void(int)> &fn) {
+  if (n <= 0) return;
+  if (threads_.empty() || n == 1) {
+    for (int i = 0; i < n; ++i) fn(i);
+    return;
+  }
+  std::unique_lock<std::mutex> lk(mu_);
+  fn_ = &fn;
+  n_ = n;
+  next_ = 0;
+  ++gen_;
+  ++active_;   // the calling thread takes part
+  cv_.notify_all();
+  while (next_ < n_) {
+    const int i = next_++;
+    lk.unlock();
+    fn(i);
+    lk.lock();
+  }
+  --active_;
+  done_cv_.wait(lk, [&] { return active_ == 0 && next_ >= n_; });
+  fn_ = nullptr;
+  n_ = 0;
+}
+
+}  // namespace wx

@@ -1,0 +1,178 @@
+// wx_runtime.h -- host-side pieces shared by the single-mesh context
+// (wx_host.cpp) and the batched ensemble (wx_ensemble.cpp):
+//   Mesh     the host tree of one AMR mesh (the reference's lp/leaf/icoord,
+//            1-based block IDs), its ghost list and the gather tables
+//            compiled from it, and the admesh tree logic (wx_mesh.cpp);
+//   Runtime  one HIP stream with pinned double-buffered upload staging and
+//            per-kernel event timing (wx_runtime.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstddef>
+#include <functional>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "wx_device.h"
+
+namespace wx {
+
+// Records the message of the last failure (wx_last_error) and returns code.
+int fail(int code, const char *fmt, ...);
+const std::string &last_error();
+
+#define HIPCHK(x)                                                                         \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) return ::wx::fail(WX_EHIP, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+template <class T>
+struct DevBuf {
+  T *ptr = nullptr;
+  size_t n = 0;
+  int ensure(size_t want) {
+    if (want <= n && ptr) return WX_OK;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    size_t cap = want > 1 ? want : 1;
+    if (hipMalloc(&ptr, cap * sizeof(T)) != hipSuccess) {
+      ptr = nullptr;
+      n = 0;
+      return fail(WX_ENOMEM, "hipMalloc of %zu bytes failed", cap * sizeof(T));
+    }
+    n = cap;
+    return WX_OK;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    n = 0;
+  }
+};
+
+// ---------------------------------------------------------------------
+// One AMR mesh on the host (src/globals.f90:13-29 for one rank).
+struct Mesh {
+  wx_params p{};
+  int cap = 0;   // largest block ID
+  int neq = 0;
+  std::vector<int32_t> lp, leaf, icoord;   // lp[nb*8 + k-1], leaf[nb], icoord[nb*2 + d]
+  int nbmax = 0, nbleafs = 0;
+  std::vector<int32_t> bounds;   // the ordered ghost list, [e*3 + c]
+  int ninternal = 0;
+  // gather tables compiled from the list
+  std::vector<LeafRec> leaves;
+  std::vector<BlockRec> binfo;
+  std::vector<int2> opsII, opsI;
+  std::vector<int> lidx;
+  bool tables_valid = false;     // leaves / binfo / ops match the tree and the list
+  bool bounds_external = false;  // the list came from the host (wx_set_bounds)
+  // block statistics (CFL partials + refinement flag) are kept per block ID;
+  // after a mesh change only the new leaves need them recomputed
+  bool stats_all_stale = true;
+  std::vector<int> new_ids;
+  // admesh scratch flags per block ID (all zero between calls)
+  std::vector<uint8_t> up, dn;
+
+  int LP(int nb, int k) const { return lp[(size_t)nb * 8 + k - 1]; }
+  void SETLP(int nb, int k, int v) { lp[(size_t)nb * 8 + k - 1] = v; }
+  void init(const wx_params &pp);
+};
+
+bool params_ok(const wx_params *p, std::string &why);
+// src/locatebounds.f90:29-540 (MPIP forms, one rank) from the tree.
+void locatebounds(Mesh *m);
+// The ordered list -> per-leaf gather ops (last writer of every segment).
+int compile_tables(Mesh *m);
+// locatebounds (unless the list is external) + compile_tables when stale.
+int refresh_tables(Mesh *m);
+// src/admesh.f90:4-111 from per-leaf flags (flag of leaves[l] at flags[l]:
+// 1 refine, 2 coarsen); appends the data ops and records the new leaves.
+int admesh_from_flags(Mesh *m, const int8_t *flag_of_leaf, std::vector<MeshOp> &ops);
+// Generalised root layout of src/basegrid.f90:26-97 (nbrootx x nbrooty roots).
+int basegrid_tree(Mesh *m);
+// Tree from the reference's Fortran arrays; *changed tells whether it differs.
+int set_tree(Mesh *m, const int32_t *lp, const int32_t *leaf, const int32_t *icoord, int32_t ld,
+             int32_t nbmax, int32_t nbleafs, bool *changed);
+int get_tree(const Mesh *m, int32_t *lp, int32_t *leaf, int32_t *icoord, int32_t ld, int32_t *nbmax,
+             int32_t *nbleafs);
+
+// ---------------------------------------------------------------------
+// Kernel IDs for the profiling counters; slots 6 and 7 are host wall time.
+enum { KID_STAGEA = 0, KID_STAGEB = 1, KID_FILLI = 2, KID_DTRED = 3, KID_STAT = 4, KID_MESH = 5,
+       HID_ADMESH = 6, HID_TABLES = 7, NPROF = 8 };
+
+struct Runtime {
+  hipStream_t stream = nullptr;
+  // pinned host staging for uploads: two buffers, each guarded by an event,
+  // so a table upload never waits for the device
+  struct Stage { char *buf = nullptr; size_t cap = 0, used = 0; hipEvent_t ev = nullptr; bool pending = false; };
+  Stage stage[2];
+  int stage_cur = 0;
+  // per-kernel timing with HIP events on this stream
+  bool prof = false;
+  struct Mark { int kid; hipEvent_t a, b; };
+  std::vector<Mark> marks;
+  std::vector<hipEvent_t> evpool;
+  double prof_ms[NPROF] = {0};
+  int64_t prof_n[NPROF] = {0};
+
+  int create_stream();
+  void destroy();
+};
+
+int stage_begin(Runtime *r, size_t total);
+int stage_copy(Runtime *r, void *dst, const void *src, size_t bytes);
+// Reserves `bytes` of the open staging buffer for the caller to fill
+// (nullptr if it does not fit); stage_push enqueues its copy to dst.
+char *stage_take(Runtime *r, size_t bytes);
+int stage_push(Runtime *r, void *dst, const char *host, size_t bytes);
+void stage_end(Runtime *r);
+void prof_collect(Runtime *r);
+int prof_report(Runtime *r, int32_t enable, double *ms, int64_t *counts, int32_t n);
+
+struct ProfScope {
+  Runtime *r;
+  int kid;
+  hipEvent_t a = nullptr;
+  ProfScope(Runtime *r_, int kid_);
+  ~ProfScope();
+};
+
+struct HostTimer {
+  Runtime *r;
+  int slot;
+  std::chrono::steady_clock::time_point t0;
+  HostTimer(Runtime *r_, int slot_) : r(r_), slot(slot_), t0(std::chrono::steady_clock::now()) {}
+  ~HostTimer();
+};
+
+// A fixed set of host worker threads for per-instance work (table
+// compilation, admesh, staging).  run(n, fn) calls fn(i) for i in [0, n)
+// across the workers and the calling thread and returns when all are done.
+class WorkerPool {
+ public:
+  explicit WorkerPool(int nthreads);
+  ~WorkerPool();
+  int size() const { return (int)threads_.size() + 1; }
+  void run(int n, const std::function<void(int)> &fn);
+
+ private:
+  void loop();
+  std::vector<std::thread> threads_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_cv_;
+  const std::function<void(int)> *fn_ = nullptr;
+  int n_ = 0, next_ = 0, active_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
+
+}  // namespace wx

@@ -229,6 +229,44 @@ def config_c3(**kw) -> Config:
     return c
 
 
+def config_c4(i: int, **kw) -> Config:
+    """Instance i (0..63) of the batched explosion ensemble: a C2-type
+    supernova in a 40 pc box with 4x4 roots and DMC cooling, with per-instance
+    draws from seed 20261202 + i in this order: log10 E ~ N(51, 0.5),
+    log10 n ~ U(-1, 1), centre x, y ~ U(0.25, 0.75) (box units),
+    max level ~ U{3, 4, 5, 6}."""
+    rng = np.random.default_rng(20261202 + i)
+    loge = rng.normal(51.0, 0.5)
+    logn = rng.uniform(-1.0, 1.0)
+    xc = rng.uniform(0.25, 0.75)
+    yc = rng.uniform(0.25, 0.75)
+    lmax = int(rng.integers(3, 7))
+    c = Config(name=f"c4_sn_{i}", neq=4, nlevs=lmax, nbrootx=4, nbrooty=4,
+               xmax=1.0, ymax=1.0, xphys=40. * PC, amol=0.6, cooling="DMC",
+               bc=("OUTFLOW",) * 4, tol=0.1, nblocks={3: 1200, 4: 3000, 5: 8000, 6: 16000}[lmax], ic="sn",
+               ic_args=dict(n=10.0 ** logn, T=1.e4, E=10.0 ** loge, R=1.0 * PC, xc=xc, yc=yc))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def config_c5(k: int, **kw) -> Config:
+    """Jet variant k of the cooling-surrogate data set: the C3 jet with seed
+    777 + k drawing v0 ~ U(150, 250) km/s, then log10 n ~ U(log10 30,
+    log10 300) cm^-3 for both the ambient medium and the jet; the velocity
+    amplitude stays a quarter of v0."""
+    rng = np.random.default_rng(777 + k)
+    v0 = rng.uniform(150.e5, 250.e5)
+    n = 10.0 ** rng.uniform(math.log10(30.), math.log10(300.))
+    c = config_c3()
+    c.name = f"c5_jet_{k}"
+    c.ic_args = dict(c.ic_args, n=n)
+    c.jet = dict(c.jet, v0=v0, dv=0.25 * v0, den=n)
+    for key, v in kw.items():
+        setattr(c, key, v)
+    return c
+
+
 CONFIGS = {"c1": config_c1, "c2": config_c2, "c3": config_c3}
 
 
