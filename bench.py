@@ -1,5 +1,8 @@
 #!/usr/bin/env python3
-"""Throughput of the plain FP64 hot path on one MI355X, in zone-updates/s.
+"""Throughput of the hot path on one MI355X, in zone-updates/s: the plain
+FP64 path on one mesh, and the batched path (many jet variants in one
+ensemble, cooling surrogate, autotuned persistent launches in hipGraphs)
+at a fixed relative L1 density error against the plain FP64 path.
 
 A step is one iteration of the reference's time loop (src/main.f90:87-98):
 CFL timestep, time += dt, tstep (stage A, ghost fill II, stage B + viscosity,
@@ -8,12 +11,17 @@ stays resident in HBM.  A zone-update is one interior leaf cell advanced by
 one step (ghost cells not counted).  The timed region is K steps after W
 warm-up steps, bracketed by device synchronisation.
 
-Prints one JSON line (see the keys below).  Options:
+Prints one JSON line: `value` is the batched path's throughput (the plain
+path's with --path plain), `plain` holds the plain path's figures,
+`accuracy` the L1 density error of the batched path's first instances
+against plain FP64 runs of the same variants.  Options:
   --gpus N          only N = 1 is supported (one GPU holds every leaf block)
   --steps K --warmup W   default K=12000, W=50: the cooling jet from t=0 to
                          about 300 yr (the configured end time), i.e. the
                          whole run with its growing AMR mesh
   --workload NAME   c3 (default: the variable cooling jet), c2, c1
+  --batch B         jet variants in the ensemble (default 16)
+  --path plain      the plain path only
   --no-cpu-baseline skip the CPU oracle timing
 """
 from __future__ import annotations
@@ -72,41 +80,9 @@ def cpu_baseline(cfg, seconds: float = 12.0):
                       f"{el:.1f} s), C oracle built with gcc -O2, one thread"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=12000)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="c3")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: --gpus {args.gpus} is not supported: this build runs every leaf block on one "
-              "GPU (multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
-
-    from walicxe2d_amd import params as P
-    from walicxe2d_amd.capi import Simulation
-
-    cfg = P.CONFIGS[args.workload]()
-    sim = Simulation(cfg)
-    sim.basegrid()
-    amr = cfg.nlevs > 1
-    if amr:
-        for _ in range(4):            # src/main.f90:60-63
-            sim.admesh()
-    if args.warmup:
-        sim.step(args.warmup, admesh=amr)
-    sim.profile(1)                    # reset + record kernel events from here
-    sim.step(0)                       # synchronise
-    t0 = time.perf_counter()
-    _, _, zones = sim.step(args.steps, admesh=amr)   # returns after a device sync
-    el = time.perf_counter() - t0
-    prof = sim.profile(0)
-    nbmax, nleaf, _ = sim.info()
-    value = zones / el
-
-    neq = cfg.neq
+def roofline(prof, zones, neq, workload, peak=HBM_PEAK_GBS):
+    """Roofline object of the dominant stage kernel: algorithmic bytes of its
+    launches over its HIP-event time on the library's stream."""
     dom = max(ALG_PASSES, key=lambda k: prof[k][0])
     ms, launches = prof[dom]
     alg_bytes = ALG_PASSES[dom] * neq * 8 * zones         # over all launches of the kernel
@@ -118,34 +94,153 @@ def main():
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
-            bpz = json.load(fh).get("per_zone_update", {}).get(f"{dom}:{args.workload}")
+            bpz = json.load(fh).get("per_zone_update", {}).get(f"{dom}:{workload}")
         if bpz is not None:
             traffic = round(bpz * zones / max(launches, 1))
-    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": peak, "unit": "GB/s",
+            "frac": round(achieved / peak, 4), "traffic": traffic,
             "kernel": dom, "kernel_ms_per_launch": round(ms / max(launches, 1), 5),
             "alg_bytes_per_launch": round(alg_bytes / max(launches, 1)),
-            "step_frac": round(value * 5 * neq * 8 / (HBM_PEAK_GBS * 1e9), 4),
             "kernels_ms": {k: round(v[0], 3) for k, v in prof.items() if v[1]}}
+
+
+def run_plain(cfg, steps, warmup):
+    """The plain FP64 path on one mesh: (zone-updates/s, ms/step, profile, final state)."""
+    from walicxe2d_amd.capi import Simulation
+    sim = Simulation(cfg)
+    sim.basegrid()
+    amr = cfg.nlevs > 1
+    if amr:
+        for _ in range(4):            # src/main.f90:60-63
+            sim.admesh()
+    if warmup:
+        sim.step(warmup, admesh=amr)
+    sim.profile(1)                    # reset + record kernel events from here
+    sim.step(0)                       # synchronise
+    t0 = time.perf_counter()
+    _, _, zones = sim.step(steps, admesh=amr)   # returns after a device sync
+    el = time.perf_counter() - t0
+    prof = sim.profile(0)
+    state = _final_state(sim)
+    nleaf = sim.info()[1]
+    sim.close()
+    return zones / el, el * 1e3 / max(steps, 1), prof, zones, state, nleaf
+
+
+def _final_state(sim):
+    u = sim.download()
+    lp, leaf, ic, _, _ = sim.tree()
+    return u, lp, leaf, ic
+
+
+def run_batched(cfgs, steps, warmup, model):
+    """The batched path: every instance in one ensemble, surrogate cooling
+    (when a model is given), autotuned persistent launches captured in
+    hipGraphs."""
+    from walicxe2d_amd.capi import Ensemble
+    ens = Ensemble(cfgs)
+    ens.basegrid()
+    for _ in range(4):
+        ens.admesh()
+    if model is not None:
+        ens.set_surrogate(model)
+    if warmup:
+        ens.step(warmup)
+    grids, threads, _ = ens.tune(reps=3)
+    ens.set_graph(True)
+    ens.step(0)
+    ens.sur_counters(reset=True)
+    t0 = time.perf_counter()
+    zones = ens.step(steps)
+    el = time.perf_counter() - t0
+    cells, fb = ens.sur_counters()
+    # kernel times of the same path, eager launches with events, a short window
+    ens.profile(1)
+    pz = ens.step(min(steps, 200))
+    prof = ens.profile(0)
+    states = []
+    for i in range(min(2, len(cfgs))):
+        u = ens.download(i)
+        lp, leaf, ic, _, _ = ens.tree(i)
+        states.append((u, lp, leaf, ic))
+    nleaf = ens.info()[1]
+    ens.close()
+    return {"value": zones / el, "ms_per_step": el * 1e3 / max(steps, 1), "zones": zones, "prof": prof,
+            "prof_zones": pz, "grids": grids, "threads": threads, "sur_cells": cells, "sur_fallbacks": fb, "states": states,
+            "nleaf": nleaf}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=12000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="c3")
+    ap.add_argument("--batch", type=int, default=16, help="jet variants in the batched ensemble")
+    ap.add_argument("--path", choices=["both", "plain"], default="both")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported: this build runs every leaf block on one "
+              "GPU (multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from walicxe2d_amd import params as P
+
+    cfg = P.CONFIGS[args.workload]()
+    neq = cfg.neq
+    pv, pms, pprof, pzones, _, pleaf = run_plain(cfg, args.steps, args.warmup)
+    plain = {"value": pv, "ms_per_step": pms, "leaf_blocks_end": pleaf,
+             "roofline": roofline(pprof, pzones, neq, args.workload),
+             "step_frac": round(pv * 5 * neq * 8 / (HBM_PEAK_GBS * 1e9), 4)}
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": pv,
         "unit": "zone-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": el * 1e3 / max(args.steps, 1),
+        "ms_per_step": pms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": cfg.name, "path": "plain-fp64", "roots": [cfg.nbrootx, cfg.nbrooty],
-                   "nlevs": cfg.nlevs, "neq": neq, "cooling": cfg.cooling, "leaf_blocks_end": nleaf,
-                   "block": "16x16+2 ghosts", "admesh_every_step": amr},
-        "roofline": roof,
+                   "nlevs": cfg.nlevs, "neq": neq, "cooling": cfg.cooling, "leaf_blocks_end": pleaf,
+                   "block": "16x16+2 ghosts", "admesh_every_step": cfg.nlevs > 1},
+        "roofline": plain["roofline"],
     }
-    sim.close()
+    if args.path == "both" and args.workload == "c3":
+        from walicxe2d_amd import surrogate as S
+        from walicxe2d_amd.l1 import l1_density
+        model = S.SurrogateModel.load() if os.path.exists(S.MODEL_PATH) else None
+        cfgs = [P.config_c5(100 + k) for k in range(args.batch)]
+        b = run_batched(cfgs, args.steps, args.warmup, model)
+        # accuracy: the first instances against plain FP64 runs of the same
+        # variants over the same steps (the plain path equals the oracle to
+        # 1e-12 per cell per step)
+        l1 = []
+        for i, st in enumerate(b["states"]):
+            _, _, _, _, ref, _ = run_plain(cfgs[i], args.steps, args.warmup)
+            l1.append(l1_density(cfgs[i], st, ref))
+        out["value"] = b["value"]
+        out["ms_per_step"] = b["ms_per_step"]
+        out["dtype"] = "f64 hydro, bf16/fp32 cooling surrogate" if model is not None else "f64"
+        out["config"].update({"path": "batched+surrogate" if model is not None else "batched",
+                              "instances": args.batch, "variants": "jet seeds 777+100..777+%d" % (99 + args.batch),
+                              "leaf_blocks_end_total": b["nleaf"], "persistent_grids": b["grids"],
+                              "threads_per_workgroup": b["threads"],
+                              "graph": True})
+        out["roofline"] = roofline(b["prof"], b["prof_zones"], neq, "c3_batched")
+        out["accuracy"] = {"l1_density_rel_vs_fp64": max(l1) if l1 else None, "instances_checked": len(l1),
+                           "bound": 1e-3,
+                           "surrogate_fallback_fraction": (b["sur_fallbacks"] / b["sur_cells"])
+                           if b["sur_cells"] else None}
+        out["plain"] = plain
+        out["speedup_vs_plain"] = b["value"] / pv
+    else:
+        out["step_frac"] = plain["step_frac"]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(P.CONFIGS[args.workload]())
     print(json.dumps(out), flush=True)

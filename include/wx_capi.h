@@ -186,16 +186,17 @@ int wx_ens_get_tree(wx_ens *ens, int32_t inst, int32_t *lp, int32_t *leaf, int32
                     int32_t *nbmax, int32_t *nbleafs);
 /* hipGraph capture of the step's launches (1 on, 0 off). */
 int wx_ens_set_graph(wx_ens *ens, int32_t enable);
-/* Workgroups of stage kernel k (0 stage A, 1 stage B, 2 boundaryI+cooling);
- * 0 = one per queue entry. */
-int wx_ens_set_grid(wx_ens *ens, int32_t kernel, int32_t grid);
-/* Autotuner: times the three stage kernels at several launch shapes on the
- * current meshes (reps launches each) and keeps the fastest; returns the
- * chosen grids and their microseconds per launch.  Later steps give the same
- * results as untuned ones. */
-int wx_ens_tune(wx_ens *ens, int32_t reps, int32_t *grids, double *us);
+/* Launch shape of stage kernel k (0 stage A, 1 stage B, 2 boundaryI+cooling):
+ * persistent workgroups (0 = one per queue entry) and threads per workgroup
+ * (256, 384 or 512). */
+int wx_ens_set_launch(wx_ens *ens, int32_t kernel, int32_t grid, int32_t threads);
+/* Autotuner: times the three stage kernels at every workgroup size and
+ * several persistent grids on the current meshes (reps launches each) and
+ * keeps the fastest; returns the chosen grids, threads and microseconds per
+ * launch (3 each).  Later steps give the same results as untuned ones. */
+int wx_ens_tune(wx_ens *ens, int32_t reps, int32_t *grids, int32_t *threads, double *us);
 int wx_ens_profile(wx_ens *ens, int32_t enable, double *ms, int64_t *counts, int32_t n);
-int wx_ens_info(wx_ens *ens, int32_t *ninst, int32_t *nleaf_total, int32_t *grids);
+int wx_ens_info(wx_ens *ens, int32_t *ninst, int32_t *nleaf_total, int32_t *grids, int32_t *threads);
 int wx_last_error(char *buf, int32_t len);
 /* HIP stream the context launches on (hipStream_t as void*). */
 void *wx_stream(wx_ctx *ctx);

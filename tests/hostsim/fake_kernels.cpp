@@ -41,14 +41,14 @@ int launch_build_queue(const QueueHdr *hdr, const int32_t *inst_off, const int32
   }
   return 0;
 }
-int launch_stageA(const wx_params *, int, QueueRef, int, const double *, double *, DevScalars *, hipStream_t) {
+int launch_stageA(const wx_params *, int, QueueRef, int, int, const double *, double *, DevScalars *, hipStream_t) {
   return 0;
 }
-int launch_stageB(const wx_params *, int, QueueRef, int, const int2 *, const BlockRec *, double *, const double *,
+int launch_stageB(const wx_params *, int, QueueRef, int, int, const int2 *, const BlockRec *, double *, const double *,
                   DevScalars *, hipStream_t) {
   return 0;
 }
-int launch_fillI_cool(const wx_params *, int, QueueRef q, int, const int2 *, const BlockRec *, const double *,
+int launch_fillI_cool(const wx_params *, int, QueueRef q, int, int, const int2 *, const BlockRec *, const double *,
                       double *, double *stat, int8_t *flag, DevScalars *, hipStream_t) {
   write_flags(q.rec, q.hdr->nleaf, (signed char *)flag, stat);
   return 0;
@@ -77,10 +77,10 @@ int launch_meshops(int, const MeshOp *ops, int nops, double *, hipStream_t) {
   return 0;
 }
 int launch_transpose(int, int, const double *, double *, int, int, hipStream_t) { return 0; }
-int launch_fillI_cool_sur(const wx_params *P, int neq, QueueRef q, int grid, const int2 *ops, const BlockRec *binfo,
-                          const double *Uin, double *Uout, double *stat, int8_t *flag, DevScalars *S, SurDev *,
-                          hipStream_t st) {
-  return launch_fillI_cool(P, neq, q, grid, ops, binfo, Uin, Uout, stat, flag, S, st);
+int launch_fillI_cool_sur(const wx_params *P, int neq, QueueRef q, int grid, int bs, const int2 *ops,
+                          const BlockRec *binfo, const double *Uin, double *Uout, double *stat, int8_t *flag,
+                          DevScalars *S, SurDev *, hipStream_t st) {
+  return launch_fillI_cool(P, neq, q, grid, bs, ops, binfo, Uin, Uout, stat, flag, S, st);
 }
 int sur_upload(SurDev *, const wx_sur_model *, const wx_params *, hipStream_t) { return 0; }
 int sur_counters(SurDev *, hipStream_t, int, int64_t *c, int64_t *f) {

@@ -62,7 +62,7 @@ def _run_pair(cfgs, nsteps, label):
             s.step(1, admesh=c.nlevs > 1)
         for i, s in enumerate(sims):
             _compare_instance(ens, i, s, f"{label} step {step} instance {i}")
-    info = ens.info()
+    info = ens.info()[:2]
     ens.close()
     for s in sims:
         s.close()
@@ -73,7 +73,7 @@ def test_explosion_ensemble_matches_single_runs():
     """Five explosion instances of different depths (3..6 levels), admesh
     every step: each equals its own single-mesh run bit for bit."""
     cfgs = [_c4(i) for i in C4_PICK]
-    n, nleaf, _ = _run_pair(cfgs, 12, "c4")
+    n, nleaf = _run_pair(cfgs, 12, "c4")
     assert n == len(cfgs) and nleaf > 0
 
 
@@ -126,8 +126,9 @@ def test_autotuned_launch_shapes_keep_results():
     b = _ens(cfgs)
     a.step(4)
     b.step(4)
-    grids, us = b.tune(reps=2)
+    grids, threads, us = b.tune(reps=2)
     assert len(grids) == 3 and all(g > 0 for g in grids) and all(t > 0 for t in us)
+    assert all(t in (256, 384, 512) for t in threads)
     b.set_graph(True)
     a.step(6)
     b.step(6)
@@ -136,16 +137,34 @@ def test_autotuned_launch_shapes_keep_results():
     b.close()
 
 
-def test_persistent_grid_smaller_than_queue():
+@pytest.mark.parametrize("threads", [256, 384, 512])
+def test_launch_shapes_keep_results(threads):
     """Stage kernels with far fewer workgroups than leaf blocks (each
-    workgroup loops over the queue) give the same states."""
+    workgroup loops over the queue) and 4, 6 or 8 waves per workgroup give
+    the same states as one 4-wave workgroup per block."""
     cfgs = [_c4(i) for i in C4_PICK[:2]]
     a = _ens(cfgs)
     b = _ens(cfgs)
     for k, g in enumerate((7, 13, 5)):
-        b.set_grid(k, g)
+        b.set_launch(k, g, threads)
     a.step(5)
     b.step(5)
-    _same_state(a, b, len(cfgs), "small grids")
+    _same_state(a, b, len(cfgs), f"launch shapes {threads}")
+    a.close()
+    b.close()
+
+
+def test_jet_launch_shapes_keep_results():
+    """The same on the jet (jet source, coolingh, reflective walls) with 6- and
+    8-wave workgroups, where stage B gathers its ghost ring in waves of its own."""
+    cfgs = [P.config_c5(k, nbrootx=16, nbrooty=4, nblocks=4000) for k in range(2)]
+    a = _ens(cfgs)
+    b = _ens(cfgs)
+    b.set_launch(0, 0, 512)
+    b.set_launch(1, 0, 384)
+    b.set_launch(2, 0, 384)
+    a.step(6)
+    b.step(6)
+    _same_state(a, b, len(cfgs), "jet launch shapes")
     a.close()
     b.close()

@@ -29,7 +29,7 @@ EXPORTS = [
     "wx_upload_tree", "wx_basegrid", "wx_admesh", "wx_step", "wx_set_time", "wx_info",
     "wx_last_error", "wx_stream", "wx_profile",
     "wx_ens_create", "wx_ens_destroy", "wx_ens_basegrid", "wx_ens_admesh", "wx_ens_step", "wx_ens_instance",
-    "wx_ens_download", "wx_ens_get_tree", "wx_ens_set_graph", "wx_ens_set_grid", "wx_ens_tune",
+    "wx_ens_download", "wx_ens_get_tree", "wx_ens_set_graph", "wx_ens_set_launch", "wx_ens_tune",
     "wx_ens_profile", "wx_ens_info",
     "wx_sur_pairs", "wx_sur_features", "wx_sur_infer", "wx_sur_grad", "wx_sur_train", "wx_set_surrogate",
     "wx_ens_set_surrogate", "wx_sur_counters", "wx_ens_sur_counters",
@@ -93,10 +93,10 @@ def load_library(path: str = LIBPATH) -> ctypes.CDLL:
     L.wx_ens_download.argtypes = [P, ctypes.c_int32, _dp]
     L.wx_ens_get_tree.argtypes = [P, ctypes.c_int32, _ip, _ip, _ip, ctypes.c_int32, _ip, _ip]
     L.wx_ens_set_graph.argtypes = [P, ctypes.c_int32]
-    L.wx_ens_set_grid.argtypes = [P, ctypes.c_int32, ctypes.c_int32]
-    L.wx_ens_tune.argtypes = [P, ctypes.c_int32, _ip, _dp]
+    L.wx_ens_set_launch.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+    L.wx_ens_tune.argtypes = [P, ctypes.c_int32, _ip, _ip, _dp]
     L.wx_ens_profile.argtypes = [P, ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]
-    L.wx_ens_info.argtypes = [P, _ip, _ip, _ip]
+    L.wx_ens_info.argtypes = [P, _ip, _ip, _ip, _ip]
     SM = ctypes.POINTER(WxSurModel)
     L.wx_sur_pairs.argtypes = [ctypes.POINTER(WxParams), _dp, ctypes.c_int64, _dp]
     L.wx_sur_features.argtypes = [ctypes.POINTER(WxParams), _dp, ctypes.c_int64, _fp, _fp]
@@ -374,22 +374,26 @@ class Ensemble:
     def set_graph(self, on: bool):
         _check(self.L.wx_ens_set_graph(self.h, 1 if on else 0), "wx_ens_set_graph")
 
-    def set_grid(self, kernel: int, grid: int):
-        _check(self.L.wx_ens_set_grid(self.h, kernel, grid), "wx_ens_set_grid")
+    def set_launch(self, kernel: int, grid: int, threads: int = 256):
+        """Launch shape of stage kernel `kernel` (0 A, 1 B, 2 fill I + cooling):
+        persistent workgroups (0 = one per leaf block) and threads per workgroup."""
+        _check(self.L.wx_ens_set_launch(self.h, kernel, grid, threads), "wx_ens_set_launch")
 
     def tune(self, reps: int = 5):
-        """Autotunes the stage kernels' launch shapes; returns (grids, us per launch)."""
+        """Autotunes the stage kernels' launch shapes; returns (grids, threads, us per launch)."""
         g = np.zeros(3, dtype=np.int32)
+        t = np.zeros(3, dtype=np.int32)
         us = np.zeros(3)
-        _check(self.L.wx_ens_tune(self.h, reps, _i(g), _d(us)), "wx_ens_tune")
-        return [int(x) for x in g], [float(x) for x in us]
+        _check(self.L.wx_ens_tune(self.h, reps, _i(g), _i(t), _d(us)), "wx_ens_tune")
+        return [int(x) for x in g], [int(x) for x in t], [float(x) for x in us]
 
     def info(self):
-        """(instances, leaf blocks in the queue, stage grids)."""
+        """(instances, leaf blocks in the queue, stage grids, stage threads)."""
         a, b = ctypes.c_int32(), ctypes.c_int32()
         g = np.zeros(3, dtype=np.int32)
-        _check(self.L.wx_ens_info(self.h, ctypes.byref(a), ctypes.byref(b), _i(g)), "wx_ens_info")
-        return a.value, b.value, [int(x) for x in g]
+        t = np.zeros(3, dtype=np.int32)
+        _check(self.L.wx_ens_info(self.h, ctypes.byref(a), ctypes.byref(b), _i(g), _i(t)), "wx_ens_info")
+        return a.value, b.value, [int(x) for x in g], [int(x) for x in t]
 
     def set_surrogate(self, model):
         _check(self.L.wx_ens_set_surrogate(self.h, ctypes.byref(model.to_c()) if model is not None else None),

@@ -63,9 +63,10 @@ struct wx_ens : Runtime {
   std::vector<MeshOp> ops;
   std::vector<LeafRec> dirty;
   std::unique_ptr<WorkerPool> pool;
-  // launch shapes: workgroups of the persistent stage kernels (A, B, fillI);
-  // 0 = one workgroup per queue entry (eager launches only)
+  // launch shapes of the stage kernels (A, B, fillI): persistent workgroups
+  // (0 = one per queue entry, eager launches only) and threads per workgroup
   int grid[3] = {0, 0, 0};
+  int bs[3] = {256, 256, 256};
   // cooling surrogate for the coolingh instances, used when use_sur
   DevBuf<SurDev> dSur;
   bool use_sur = false;
@@ -224,20 +225,20 @@ int launch_step(wx_ens *e, int par, bool for_graph) {
   }
   if (!rc) {
     ProfScope ps(e, prof ? KID_STAGEA : -1);
-    rc = launch_stageA(e->dP.ptr, e->neq, q, stage_grid(e, 0, for_graph), Ucur, Uoth, e->dS.ptr, e->stream);
+    rc = launch_stageA(e->dP.ptr, e->neq, q, stage_grid(e, 0, for_graph), e->bs[0], Ucur, Uoth, e->dS.ptr, e->stream);
   }
   if (!rc) {
     ProfScope ps(e, prof ? KID_STAGEB : -1);
-    rc = launch_stageB(e->dP.ptr, e->neq, q, stage_grid(e, 1, for_graph), e->dOpsII.ptr, e->dBinfo.ptr, Ucur, Uoth,
+    rc = launch_stageB(e->dP.ptr, e->neq, q, stage_grid(e, 1, for_graph), e->bs[1], e->dOpsII.ptr, e->dBinfo.ptr, Ucur, Uoth,
                        e->dS.ptr, e->stream);
   }
   if (!rc) {
     ProfScope ps(e, prof ? KID_FILLI : -1);
     if (e->use_sur)
-      rc = launch_fillI_cool_sur(e->dP.ptr, e->neq, q, stage_grid(e, 2, for_graph), e->dOpsI.ptr, e->dBinfo.ptr, Ucur,
+      rc = launch_fillI_cool_sur(e->dP.ptr, e->neq, q, stage_grid(e, 2, for_graph), e->bs[2], e->dOpsI.ptr, e->dBinfo.ptr, Ucur,
                                  Uoth, e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, e->dSur.ptr, e->stream);
     else
-      rc = launch_fillI_cool(e->dP.ptr, e->neq, q, stage_grid(e, 2, for_graph), e->dOpsI.ptr, e->dBinfo.ptr, Ucur,
+      rc = launch_fillI_cool(e->dP.ptr, e->neq, q, stage_grid(e, 2, for_graph), e->bs[2], e->dOpsI.ptr, e->dBinfo.ptr, Ucur,
                              Uoth, e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, e->stream);
   }
   if (!rc) {
@@ -504,22 +505,24 @@ int wx_ens_set_graph(wx_ens *e, int32_t enable) {
   return WX_OK;
 }
 
-int wx_ens_set_grid(wx_ens *e, int32_t kernel, int32_t grid) {
+int wx_ens_set_launch(wx_ens *e, int32_t kernel, int32_t grid, int32_t threads) {
   if (!e) return fail(WX_EARG, "null ensemble");
   if (kernel < 0 || kernel > 2 || grid < 0) return fail(WX_EARG, "kernel %d / grid %d", kernel, grid);
+  if (threads != 256 && threads != 384 && threads != 512) return fail(WX_EARG, "threads %d (256, 384 or 512)", threads);
   e->grid[kernel] = grid;
+  e->bs[kernel] = threads;
   drop_graphs(e);
   return WX_OK;
 }
 
-// Launch-shape autotuner: times each stage kernel over `reps` launches at
-// several persistent grid sizes (1, 2, 4, 8 workgroups per CU, and one
-// workgroup per queue entry) on the current meshes and keeps the fastest.
-// Stage A and the fill are idempotent given Uⁿ; stage B updates in place,
-// so it runs on a scratch copy of Uⁿ.  The statistics the fill's epilogue
-// overwrites are recomputed afterwards, so tuning leaves the results of
-// later steps unchanged.
-int wx_ens_tune(wx_ens *e, int32_t reps, int32_t *grids, double *us) {
+// Launch-shape autotuner: times each stage kernel over `reps` launches for
+// every workgroup size (4, 6, 8 waves) and persistent grid (1, 2, 4, 8, 16
+// workgroups per CU, and one workgroup per queue entry) on the current
+// meshes, and keeps the fastest.  Stage A and the fill are idempotent given
+// Uⁿ; stage B updates in place, so it runs on a scratch copy of Uⁿ.  The
+// statistics the fill's epilogue overwrites are recomputed afterwards, so
+// tuning leaves the results of later steps unchanged.
+int wx_ens_tune(wx_ens *e, int32_t reps, int32_t *grids, int32_t *threads, double *us) {
   if (!e || reps < 1) return fail(WX_EARG, "bad argument");
   int rc = ensure_stats(e);
   if (rc) return rc;
@@ -540,29 +543,39 @@ int wx_ens_tune(wx_ens *e, int32_t reps, int32_t *grids, double *us) {
   cands.push_back(e->nleaf_total);
   for (int k = 0; k < 3 && !rc; ++k) {
     double best = 1e30;
-    int bestg = e->nleaf_total;
-    for (int g : cands) {
-      float tot = 0.f;
-      for (int r = 0; r <= reps && !rc; ++r) {   // r = 0 warms up
-        if (k == 1 && hipMemcpyAsync(scratch.ptr, e->cur_u(), tiles * sizeof(double), hipMemcpyDeviceToDevice,
-                                     e->stream) != hipSuccess) rc = fail(WX_EHIP, "tune copy");
-        (void)hipEventRecord(a, e->stream);
-        if (k == 0) rc = launch_stageA(e->dP.ptr, e->neq, q, g, e->cur_u(), e->other_u(), e->dS.ptr, e->stream);
-        else if (k == 1) rc = launch_stageB(e->dP.ptr, e->neq, q, g, e->dOpsII.ptr, e->dBinfo.ptr, scratch.ptr,
-                                            e->other_u(), e->dS.ptr, e->stream);
-        else rc = launch_fillI_cool(e->dP.ptr, e->neq, q, g, e->dOpsI.ptr, e->dBinfo.ptr, e->cur_u(), scratch.ptr,
-                                    e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, e->stream);
-        (void)hipEventRecord(b, e->stream);
-        (void)hipEventSynchronize(b);
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, a, b);
-        if (r > 0) tot += ms;
+    int bestg = e->nleaf_total, bestb = 256;
+    for (int bsz : {256, 384, 512}) {
+      for (int g : cands) {
+        float tot = 0.f;
+        for (int r = 0; r <= reps && !rc; ++r) {   // r = 0 warms up
+          if (k == 1 && hipMemcpyAsync(scratch.ptr, e->cur_u(), tiles * sizeof(double), hipMemcpyDeviceToDevice,
+                                       e->stream) != hipSuccess) rc = fail(WX_EHIP, "tune copy");
+          (void)hipEventRecord(a, e->stream);
+          if (k == 0)
+            rc = launch_stageA(e->dP.ptr, e->neq, q, g, bsz, e->cur_u(), e->other_u(), e->dS.ptr, e->stream);
+          else if (k == 1)
+            rc = launch_stageB(e->dP.ptr, e->neq, q, g, bsz, e->dOpsII.ptr, e->dBinfo.ptr, scratch.ptr, e->other_u(),
+                               e->dS.ptr, e->stream);
+          else if (e->use_sur)
+            rc = launch_fillI_cool_sur(e->dP.ptr, e->neq, q, g, bsz, e->dOpsI.ptr, e->dBinfo.ptr, e->cur_u(),
+                                       scratch.ptr, e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, e->dSur.ptr, e->stream);
+          else
+            rc = launch_fillI_cool(e->dP.ptr, e->neq, q, g, bsz, e->dOpsI.ptr, e->dBinfo.ptr, e->cur_u(), scratch.ptr,
+                                   e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, e->stream);
+          (void)hipEventRecord(b, e->stream);
+          (void)hipEventSynchronize(b);
+          float ms = 0.f;
+          (void)hipEventElapsedTime(&ms, a, b);
+          if (r > 0) tot += ms;
+        }
+        const double t = 1e3 * tot / reps;
+        if (t < best) { best = t; bestg = g; bestb = bsz; }
       }
-      const double t = 1e3 * tot / reps;
-      if (t < best) { best = t; bestg = g; }
     }
     e->grid[k] = bestg == e->nleaf_total ? 0 : bestg;
+    e->bs[k] = bestb;
     if (grids) grids[k] = bestg;
+    if (threads) threads[k] = bestb;
     if (us) us[k] = best;
   }
   (void)hipEventDestroy(a);
@@ -570,6 +583,7 @@ int wx_ens_tune(wx_ens *e, int32_t reps, int32_t *grids, double *us) {
   HIPCHK(hipStreamSynchronize(e->stream));
   scratch.release();
   HIPCHK(hipMemcpy(e->dS.ptr, saved.data(), (size_t)e->n * sizeof(DevScalars), hipMemcpyHostToDevice));
+  if (e->use_sur) HIPCHK(hipMemsetAsync(&e->dSur.ptr->cells, 0, 2 * sizeof(unsigned long long), e->stream));
   for (auto &m : e->m) m.stats_all_stale = true;
   drop_graphs(e);
   return rc;
@@ -609,11 +623,12 @@ int wx_ens_profile(wx_ens *e, int32_t enable, double *ms, int64_t *counts, int32
   return prof_report(e, enable, ms, counts, n);
 }
 
-int wx_ens_info(wx_ens *e, int32_t *ninst, int32_t *nleaf_total, int32_t *grids) {
+int wx_ens_info(wx_ens *e, int32_t *ninst, int32_t *nleaf_total, int32_t *grids, int32_t *threads) {
   if (!e) return fail(WX_EARG, "null ensemble");
   if (ninst) *ninst = e->n;
   if (nleaf_total) *nleaf_total = e->nleaf_total;
   if (grids) for (int k = 0; k < 3; ++k) grids[k] = e->grid[k];
+  if (threads) for (int k = 0; k < 3; ++k) threads[k] = e->bs[k];
   return WX_OK;
 }
 

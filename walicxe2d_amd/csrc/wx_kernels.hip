@@ -197,8 +197,8 @@ __device__ __forceinline__ void cyl_source(const wx_params *__restrict__ p, cons
 
 // ---------------------------------------------------------------------
 // Stage A: src/tstep.f90:22-46 (hllcfluxes choice 1 + step with dt/2).
-template <int NEQ>
-__global__ __launch_bounds__(256) void k_stageA(const wx_params *__restrict__ P, QueueRef q,
+template <int NEQ, int BS>
+__global__ __launch_bounds__(BS) void k_stageA(const wx_params *__restrict__ P, QueueRef q,
                                                 const double *__restrict__ U, double *__restrict__ Q,
                                                 DevScalars *__restrict__ Sa) {
   __shared__ double prim[NEQ * NC];
@@ -225,19 +225,21 @@ __global__ __launch_bounds__(256) void k_stageA(const wx_params *__restrict__ P,
   __syncthreads();
   int err = flux_sweep<NEQ, 1>(p, prim, F, G);
   __syncthreads();
-  const double dtm = S->dt / 2.;
-  const double dtdx = dtm / p->dx[L.level - 1];
-  const double dtdy = dtm / p->dy[L.level - 1];
-  const int i = threadIdx.x % 16 + 1, j = threadIdx.x / 16 + 1;
-  double Sc[NEQ];
-  if (p->cylindrical) cyl_source<NEQ>(p, prim, i, j, L.icy, L.level, Sc);
-  double *qb = Q + (size_t)L.id * NEQ * NC;
+  if (threadIdx.x < 256) {   // one thread per interior cell
+    const double dtm = S->dt / 2.;
+    const double dtdx = dtm / p->dx[L.level - 1];
+    const double dtdy = dtm / p->dy[L.level - 1];
+    const int i = threadIdx.x % 16 + 1, j = threadIdx.x / 16 + 1;
+    double Sc[NEQ];
+    if (p->cylindrical) cyl_source<NEQ>(p, prim, i, j, L.icy, L.level, Sc);
+    double *qb = Q + (size_t)L.id * NEQ * NC;
 #pragma unroll
-  for (int k = 0; k < NEQ; ++k) {
-    double v = ub[k * NC + IDX(i, j)] - dtdx * (F[k * 272 + (j - 1) * 17 + i] - F[k * 272 + (j - 1) * 17 + i - 1])
-               - dtdy * (G[k * 272 + j * 16 + (i - 1)] - G[k * 272 + (j - 1) * 16 + (i - 1)]);
-    if (p->cylindrical) v = v + dtm * Sc[k];
-    qb[k * NC + IDX(i, j)] = v;
+    for (int k = 0; k < NEQ; ++k) {
+      double v = ub[k * NC + IDX(i, j)] - dtdx * (F[k * 272 + (j - 1) * 17 + i] - F[k * 272 + (j - 1) * 17 + i - 1])
+                 - dtdy * (G[k * 272 + j * 16 + (i - 1)] - G[k * 272 + (j - 1) * 16 + (i - 1)]);
+      if (p->cylindrical) v = v + dtm * Sc[k];
+      qb[k * NC + IDX(i, j)] = v;
+    }
   }
   if (err) atomicOr(&S->err, WX_EHLLC);
   __syncthreads();   // the next queue entry reuses the LDS tiles
@@ -246,8 +248,8 @@ __global__ __launch_bounds__(256) void k_stageA(const wx_params *__restrict__ P,
 
 // ---------------------------------------------------------------------
 // Stage B: src/tstep.f90:53-88 with boundaryII pulled from the neighbours.
-template <int NEQ>
-__global__ __launch_bounds__(256) void k_stageB(const wx_params *__restrict__ P, QueueRef q,
+template <int NEQ, int BS>
+__global__ __launch_bounds__(BS) void k_stageB(const wx_params *__restrict__ P, QueueRef q,
                                                 const int2 *__restrict__ ops,
                                                 const BlockRec *__restrict__ binfo,
                                                 double *__restrict__ U, const double *__restrict__ Q,
@@ -271,7 +273,7 @@ __global__ __launch_bounds__(256) void k_stageB(const wx_params *__restrict__ P,
   double *ub = U + (size_t)b * NEQ * NC;
   const int tid = threadIdx.x;
   // interior of U^{n+1/2} (imposed when the block is in the source list)
-  {
+  if (tid < 256) {
     const int i = tid % 16 + 1, j = tid / 16 + 1;
     double uu[NEQ], q[NEQ], T;
 #pragma unroll
@@ -281,12 +283,14 @@ __global__ __launch_bounds__(256) void k_stageB(const wx_params *__restrict__ P,
 #pragma unroll
     for (int k = 0; k < NEQ; ++k) prim[k * NC + IDX(i, j)] = q[k];
   }
-  // two-layer ghost ring: 4 sides x 2 layers x 16 positions = 128 cells
+  // two-layer ghost ring: 4 sides x 2 layers x 16 positions = 128 cells,
+  // gathered by waves of their own when the workgroup has them
   double gh[NEQ];
   int gi = 0, gj = 0;
-  const bool ghost = tid < 128;
+  const int gt = BS >= 384 ? tid - 256 : tid;
+  const bool ghost = gt >= 0 && gt < 128;
   if (ghost) {
-    const int side = tid >> 5, layer = ((tid >> 4) & 1) + 1, pos = (tid & 15) + 1;
+    const int side = gt >> 5, layer = ((gt >> 4) & 1) + 1, pos = (gt & 15) + 1;
     const int2 op = ops[r * 8 + side * 2 + ((pos - 1) >> 3)];
     ghost_gather<NEQ>(p, Q, binfo, op, b, L.level, L.icx, L.icy, side, layer, pos, vjet, gh);
     ghost_coords(side, layer, pos, gi, gj);
@@ -298,28 +302,31 @@ __global__ __launch_bounds__(256) void k_stageB(const wx_params *__restrict__ P,
   __syncthreads();
   int err = flux_sweep<NEQ, 2>(p, prim, F, G);
   __syncthreads();
-  const double dt = S->dt;
-  const double dtdx = dt / p->dx[L.level - 1];
-  const double dtdy = dt / p->dy[L.level - 1];
   const int i = tid % 16 + 1, j = tid / 16 + 1;
-  double Sc[NEQ];
-  if (p->cylindrical) cyl_source<NEQ>(p, prim, i, j, L.icy, L.level, Sc);
-  const bool visc = L.level >= p->nlevs - 1;   // src/tstep.f90:82
-  const double eta = p->eta;
   double un[NEQ];
+  if (tid < 256) {   // one thread per interior cell
+    const double dt = S->dt;
+    const double dtdx = dt / p->dx[L.level - 1];
+    const double dtdy = dt / p->dy[L.level - 1];
+    double Sc[NEQ];
+    if (p->cylindrical) cyl_source<NEQ>(p, prim, i, j, L.icy, L.level, Sc);
+    const bool visc = L.level >= p->nlevs - 1;   // src/tstep.f90:82
+    const double eta = p->eta;
 #pragma unroll
-  for (int k = 0; k < NEQ; ++k) {
-    const double *uk = ub + k * NC;
-    double v = uk[IDX(i, j)] - dtdx * (F[k * 272 + (j - 1) * 17 + i] - F[k * 272 + (j - 1) * 17 + i - 1])
-               - dtdy * (G[k * 272 + j * 16 + (i - 1)] - G[k * 272 + (j - 1) * 16 + (i - 1)]);
-    if (p->cylindrical) v = v + dt * Sc[k];
-    if (visc)   // src/viscosity.f90:6-25 (no dt factor)
-      v = v + eta * (uk[IDX(i + 1, j)] + uk[IDX(i - 1, j)] + uk[IDX(i, j + 1)] + uk[IDX(i, j - 1)] - 4. * uk[IDX(i, j)]);
-    un[k] = v;
+    for (int k = 0; k < NEQ; ++k) {
+      const double *uk = ub + k * NC;
+      double v = uk[IDX(i, j)] - dtdx * (F[k * 272 + (j - 1) * 17 + i] - F[k * 272 + (j - 1) * 17 + i - 1])
+                 - dtdy * (G[k * 272 + j * 16 + (i - 1)] - G[k * 272 + (j - 1) * 16 + (i - 1)]);
+      if (p->cylindrical) v = v + dt * Sc[k];
+      if (visc)   // src/viscosity.f90:6-25 (no dt factor)
+        v = v + eta * (uk[IDX(i + 1, j)] + uk[IDX(i - 1, j)] + uk[IDX(i, j + 1)] + uk[IDX(i, j - 1)] - 4. * uk[IDX(i, j)]);
+      un[k] = v;
+    }
   }
   __syncthreads();   // every read of Uⁿ in this tile is done
+  if (tid < 256)
 #pragma unroll
-  for (int k = 0; k < NEQ; ++k) ub[k * NC + IDX(i, j)] = un[k];
+    for (int k = 0; k < NEQ; ++k) ub[k * NC + IDX(i, j)] = un[k];
   if (ghost) {
 #pragma unroll
     for (int k = 0; k < NEQ; ++k) ub[k * NC + IDX(gi, gj)] = gh[k];
@@ -337,12 +344,12 @@ __global__ __launch_bounds__(256) void k_stageB(const wx_params *__restrict__ P,
 template <int NEQ>
 __device__ void block_stats(const wx_params *__restrict__ p, const double *prim, const LeafRec &L,
                             double *stat, int8_t *flag) {
-  __shared__ double red[3][4];
+  __shared__ double red[3][16];
   const int tid = threadIdx.x;
   const int i = tid % 16 + 1, j = tid / 16 + 1;
   const int lev = L.level;
   double dtp = 1.e30, dtc = 1e30, grad = 0.;
-  {
+  if (tid < 256) {   // one thread per interior cell
     const double rho = prim[0 * NC + IDX(i, j)], vx = prim[1 * NC + IDX(i, j)];
     const double vy = prim[2 * NC + IDX(i, j)], pr = prim[3 * NC + IDX(i, j)];
     const double cs = sqrt(p->gamma * pr / rho);
@@ -372,7 +379,7 @@ __device__ void block_stats(const wx_params *__restrict__ p, const double *prim,
   if ((tid & 63) == 0) { red[0][w] = dtp; red[1][w] = dtc; red[2][w] = grad; }
   __syncthreads();
   if (tid == 0) {
-    for (int k = 1; k < 4; ++k) {
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
       dtp = dmin(dtp, red[0][k]);
       dtc = dmin(dtc, red[1][k]);
       grad = dmax(grad, red[2][k]);
@@ -421,7 +428,7 @@ __device__ void cool_surrogate(const wx_params *__restrict__ p, SurDev *__restri
   __syncthreads();
   {
     const int l = tid & 63, r = l & 15, qq = l >> 4;
-    for (int g = tid >> 6; g < 21; g += 4) {   // 21 groups of 16 cover the 324 cells
+    for (int g = tid >> 6; g < 21; g += (int)(blockDim.x >> 6)) {   // 21 groups of 16 cover the 324 cells
       const int c = 16 * g + r;
       bf16x4 xb = {0, 0, 0, 0};
       if (qq == 0 && c < 324) xb = pack4(sx[4 * c], sx[4 * c + 1], sx[4 * c + 2], sx[4 * c + 3]);
@@ -475,8 +482,8 @@ __device__ void cool_surrogate(const wx_params *__restrict__ p, SurDev *__restri
   }
 }
 
-template <int NEQ, bool SUR>
-__global__ __launch_bounds__(256) void k_fillI_cool(const wx_params *__restrict__ P, QueueRef q,
+template <int NEQ, bool SUR, int BS>
+__global__ __launch_bounds__(BS) void k_fillI_cool(const wx_params *__restrict__ P, QueueRef q,
                                                     const int2 *__restrict__ ops,
                                                     const BlockRec *__restrict__ binfo,
                                                     const double *__restrict__ Uin,
@@ -791,36 +798,45 @@ int launch_build_queue(const QueueHdr *hdr, const int32_t *inst_off, const int32
   return hip_status();
 }
 
-int launch_stageA(const wx_params *P, int neq, QueueRef q, int grid, const double *U, double *Q, DevScalars *S,
-                  hipStream_t stream) {
+// Workgroup sizes the stage kernels are built for (4, 6 or 8 waves).
+#define WX_BS_DISPATCH(BSV, CALL)                    \
+  switch (BSV) {                                     \
+    case 256: { constexpr int B = 256; CALL; } break; \
+    case 384: { constexpr int B = 384; CALL; } break; \
+    case 512: { constexpr int B = 512; CALL; } break; \
+    default: return WX_EARG;                         \
+  }
+
+int launch_stageA(const wx_params *P, int neq, QueueRef q, int grid, int bs, const double *U, double *Q,
+                  DevScalars *S, hipStream_t stream) {
   if (grid <= 0) return WX_OK;
-  WX_DISPATCH(neq, hipLaunchKernelGGL(k_stageA<N>, dim3(grid), dim3(256), 0, stream, P, q, U, Q, S));
+  WX_BS_DISPATCH(bs, WX_DISPATCH(neq, hipLaunchKernelGGL((k_stageA<N, B>), dim3(grid), dim3(B), 0, stream, P, q, U, Q, S)));
   return hip_status();
 }
 
-int launch_stageB(const wx_params *P, int neq, QueueRef q, int grid, const int2 *ops, const BlockRec *binfo,
+int launch_stageB(const wx_params *P, int neq, QueueRef q, int grid, int bs, const int2 *ops, const BlockRec *binfo,
                   double *U, const double *Q, DevScalars *S, hipStream_t stream) {
   if (grid <= 0) return WX_OK;
-  WX_DISPATCH(neq, hipLaunchKernelGGL(k_stageB<N>, dim3(grid), dim3(256), 0, stream, P, q, ops, binfo, U, Q, S));
+  WX_BS_DISPATCH(bs, WX_DISPATCH(neq, hipLaunchKernelGGL((k_stageB<N, B>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, U, Q, S)));
   return hip_status();
 }
 
-int launch_fillI_cool(const wx_params *P, int neq, QueueRef q, int grid, const int2 *ops, const BlockRec *binfo,
-                      const double *Uin, double *Uout, double *stat, int8_t *flag, DevScalars *S,
-                      hipStream_t stream) {
+int launch_fillI_cool(const wx_params *P, int neq, QueueRef q, int grid, int bs, const int2 *ops,
+                      const BlockRec *binfo, const double *Uin, double *Uout, double *stat, int8_t *flag,
+                      DevScalars *S, hipStream_t stream) {
   if (grid <= 0) return WX_OK;
-  WX_DISPATCH(neq, hipLaunchKernelGGL((k_fillI_cool<N, false>), dim3(grid), dim3(256), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, nullptr));
+  WX_BS_DISPATCH(bs, WX_DISPATCH(neq, hipLaunchKernelGGL((k_fillI_cool<N, false, B>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, nullptr)));
   return hip_status();
 }
 
-int launch_fillI_cool_sur(const wx_params *P, int neq, QueueRef q, int grid, const int2 *ops, const BlockRec *binfo,
-                          const double *Uin, double *Uout, double *stat, int8_t *flag, DevScalars *S, SurDev *M,
-                          hipStream_t stream) {
+int launch_fillI_cool_sur(const wx_params *P, int neq, QueueRef q, int grid, int bs, const int2 *ops,
+                          const BlockRec *binfo, const double *Uin, double *Uout, double *stat, int8_t *flag,
+                          DevScalars *S, SurDev *M, hipStream_t stream) {
   if (grid <= 0) return WX_OK;
-  switch (neq) {
-    case 5: hipLaunchKernelGGL((k_fillI_cool<5, true>), dim3(grid), dim3(256), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, M); break;
-    case 6: hipLaunchKernelGGL((k_fillI_cool<6, true>), dim3(grid), dim3(256), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, M); break;
-    default: return WX_EARG;   // coolingh needs u(5) = n_HI
+  switch (neq) {   // coolingh needs u(5) = n_HI
+    case 5: WX_BS_DISPATCH(bs, hipLaunchKernelGGL((k_fillI_cool<5, true, B>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, M)); break;
+    case 6: WX_BS_DISPATCH(bs, hipLaunchKernelGGL((k_fillI_cool<6, true, B>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, M)); break;
+    default: return WX_EARG;
   }
   return hip_status();
 }

@@ -90,16 +90,16 @@ void locatebounds(Mesh *c) {
 
 // ---------------------------------------------------------------------
 // Compile the ordered list into per-segment gather ops (last writer wins,
-// jet-imposition order folded into two flags).
-struct Seg {
-  int kind = GK_NONE, src = 0, off = 0, e = -1;
-};
-
+// jet-imposition order folded into two flags).  Ops are written in place:
+// op.x = source block, op.y = kind | offset << GF_OFF_SHIFT, and the list
+// position of the writer is kept beside each op for the imposition flags.
+// All scratch lives in the Mesh and is reused from call to call.
 int compile_tables(Mesh *c) {
   const int cap = c->nbmax;   // IDs above nbmax hold no block
   c->leaves.clear();
   c->lidx.assign((size_t)cap + 1, -1);
-  c->binfo.assign((size_t)cap + 1, BlockRec{0, 0, 0, 0});
+  c->binfo.resize((size_t)cap + 1);
+  c->binfo[0] = BlockRec{0, 0, 0, 0};
   for (int nb = 1; nb <= c->nbmax; ++nb) {
     c->binfo[nb] = BlockRec{c->LP(nb, 1), c->icoord[(size_t)nb * 2], c->icoord[(size_t)nb * 2 + 1], 0};
     if (!c->leaf[nb]) continue;
@@ -114,20 +114,32 @@ int compile_tables(Mesh *c) {
     c->leaves.push_back(r);
   }
   const int nleaf = (int)c->leaves.size();
-  std::vector<Seg> s2((size_t)nleaf * 8), s1((size_t)nleaf * 8);
-  std::vector<int> ifirst((size_t)cap + 1, INT_MAX), ilast((size_t)cap + 1, -1);
-  auto put = [&](std::vector<Seg> &T, int dst, int side, int half, int kind, int src, int off, int e) {
+  const size_t nseg = (size_t)nleaf * 8;
+  c->opsII.assign(nseg, int2{0, GK_NONE});
+  c->opsI.assign(nseg, int2{0, GK_NONE});
+  const bool otherb = c->p.otherb != 0;
+  if (otherb) {
+    c->segE2.resize(nseg);
+    c->segE1.resize(nseg);
+    c->ifirst.assign((size_t)cap + 1, INT_MAX);
+    c->ilast.assign((size_t)cap + 1, -1);
+  }
+  int2 *o2 = c->opsII.data(), *o1 = c->opsI.data();
+  int *e2 = otherb ? c->segE2.data() : nullptr, *e1 = otherb ? c->segE1.data() : nullptr;
+  const int *lidx = c->lidx.data();
+  auto put = [&](int2 *T, int *E, int dst, int side, int half, int kind, int src, int off, int e) {
     if (dst < 1 || dst > cap) return;
-    const int li = c->lidx[dst];
+    const int li = lidx[dst];
     if (li < 0) return;   // not a leaf: its data is never read again
-    Seg &s = T[(size_t)li * 8 + side * 2 + half];
-    s.kind = kind;
-    s.src = src;
-    s.off = off;
-    s.e = e;
+    const size_t k = (size_t)li * 8 + side * 2 + half;
+    T[k] = int2{src, kind | (off << GF_OFF_SHIFT)};
+    if (E) E[k] = e;
   };
   auto both = [&](int dst, int side, int kind, int src, int off, int e) {
-    for (int h = 0; h < 2; ++h) { put(s2, dst, side, h, kind, src, off, e); put(s1, dst, side, h, kind, src, off, e); }
+    for (int h = 0; h < 2; ++h) {
+      put(o2, e2, dst, side, h, kind, src, off, e);
+      put(o1, e1, dst, side, h, kind, src, off, e);
+    }
   };
   const bool outfyb_bug = (c->p.compat & 2) && c->p.bc[2] == 0;
   for (int e = 0; e < c->ninternal; ++e) {
@@ -137,12 +149,12 @@ int compile_tables(Mesh *c) {
       const int side = -code - 1;
       if (side > 3) return fail(WX_EARG, "bound %d: code %d", e + 1, code);
       const int kind = c->p.bc[side] ? GK_BOXREF : GK_BOXOUT;
-      for (int h = 0; h < 2; ++h) put(s2, nbs, side, h, kind, nbs, 0, e);   // boundaryII.f90:38-141
+      for (int h = 0; h < 2; ++h) put(o2, e2, nbs, side, h, kind, nbs, 0, e);   // boundaryII.f90:38-141
       if (side == 2 && outfyb_bug) {
-        const int k = e + 1;                                                // boundaryI.f90:87
-        for (int h = 0; h < 2; ++h) put(s1, k, 2, h, GK_BOXOUT, k, 0, e);
+        const int k = e + 1;                                                    // boundaryI.f90:87
+        for (int h = 0; h < 2; ++h) put(o1, e1, k, 2, h, GK_BOXOUT, k, 0, e);
       } else {
-        for (int h = 0; h < 2; ++h) put(s1, nbs, side, h, kind, nbs, 0, e);
+        for (int h = 0; h < 2; ++h) put(o1, e1, nbs, side, h, kind, nbs, 0, e);
       }
     } else {
       if (n1 < 1 || n1 > c->nbmax) return fail(WX_EARG, "bound %d: neighbour %d out of range", e + 1, n1);
@@ -155,44 +167,43 @@ int compile_tables(Mesh *c) {
           const int side = (code - 2) / 3;       // 0 L, 1 R, 2 B, 3 T
           const int h = (code - 2) % 3;          // 0: first fine block, 1: second
           const int fine_side = side ^ 1;        // the fine block's facing side
-          put(s2, nbs, side, h, GK_RESTRICT, n1, 0, e);
-          put(s1, nbs, side, h, GK_RESTRICT, n1, 0, e);
+          put(o2, e2, nbs, side, h, GK_RESTRICT, n1, 0, e);
+          put(o1, e1, nbs, side, h, GK_RESTRICT, n1, 0, e);
           both(n1, fine_side, GK_PROLONG, nbs, 8 * h, e);
           break;
         }
         default: return fail(WX_EARG, "bound %d: unknown code %d", e + 1, code);
       }
     }
-    if (c->p.otherb) {   // impose_user_bc(nbs) after each entry
-      ifirst[nbs] = std::min(ifirst[nbs], e);
-      ilast[nbs] = std::max(ilast[nbs], e);
+    if (otherb) {   // impose_user_bc(nbs) after each entry
+      c->ifirst[nbs] = std::min(c->ifirst[nbs], e);
+      c->ilast[nbs] = std::max(c->ilast[nbs], e);
     }
   }
-  auto encode = [&](const std::vector<Seg> &T, std::vector<int2> &out, bool full, const char *name) -> int {
-    out.assign((size_t)nleaf * 8, int2{0, 0});
+  // every segment of fill II is written by the list (boundaryI may leave
+  // segments to the carried values)
+  for (size_t k = 0; k < nseg; ++k)
+    if ((o2[k].y & 15) == GK_NONE)
+      return fail(WX_EARG, "boundaryII: block %d ghost segment %d not covered by the list",
+                  c->leaves[k / 8].id, (int)(k % 8));
+  if (otherb) {
+    const int *ifirst = c->ifirst.data(), *ilast = c->ilast.data();
     for (int l = 0; l < nleaf; ++l) {
       const int dst = c->leaves[l].id;
       for (int s = 0; s < 8; ++s) {
-        const Seg &g = T[(size_t)l * 8 + s];
-        if (g.kind == GK_NONE) {
-          if (full) return fail(WX_EARG, "%s: block %d ghost segment %d not covered by the list", name, dst, s);
-          continue;
+        const size_t k = (size_t)l * 8 + s;
+        if ((o2[k].y & 15) != GK_NONE) {
+          if (ifirst[o2[k].x] < e2[k]) o2[k].y |= GF_SRC_IMPOSED;
+          if (ilast[dst] >= e2[k]) o2[k].y |= GF_DST_IMPOSED;
         }
-        int y = g.kind | (g.off << GF_OFF_SHIFT);
-        if (c->p.otherb) {
-          if (ifirst[g.src] < g.e) y |= GF_SRC_IMPOSED;
-          if (ilast[dst] >= g.e) y |= GF_DST_IMPOSED;
+        if ((o1[k].y & 15) != GK_NONE) {
+          if (ifirst[o1[k].x] < e1[k]) o1[k].y |= GF_SRC_IMPOSED;
+          if (ilast[dst] >= e1[k]) o1[k].y |= GF_DST_IMPOSED;
         }
-        out[(size_t)l * 8 + s] = int2{g.src, y};
       }
+      c->leaves[l].flags = ilast[dst] >= 0 ? 1 : 0;
     }
-    return WX_OK;
-  };
-  int rc = encode(s2, c->opsII, true, "boundaryII");
-  if (rc) return rc;
-  rc = encode(s1, c->opsI, false, "boundaryI");
-  if (rc) return rc;
-  for (auto &r : c->leaves) r.flags = (c->p.otherb && ilast[r.id] >= 0) ? 1 : 0;
+  }
   return WX_OK;
 }
 

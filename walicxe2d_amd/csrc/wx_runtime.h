@@ -72,6 +72,7 @@ struct Mesh {
   std::vector<BlockRec> binfo;
   std::vector<int2> opsII, opsI;
   std::vector<int> lidx;
+  std::vector<int> segE2, segE1, ifirst, ilast;   // compile_tables scratch
   bool tables_valid = false;     // leaves / binfo / ops match the tree and the list
   bool bounds_external = false;  // the list came from the host (wx_set_bounds)
   // block statistics (CFL partials + refinement flag) are kept per block ID;

@@ -103,16 +103,19 @@ __device__ __forceinline__ void layer_backward(float *g, int woff, int boff, int
     }
   if (da) {
     // da[k][n] = sum over o of W[o][k] * d[o][n]
+    bf16x4 bd[4];
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) bd[kt] = pack4(d[kt][0], d[kt][1], d[kt][2], d[kt][3]);
 #pragma unroll
     for (int mt = 0; mt < IN_TILES; ++mt) {
+      bf16x4 wa[4];
+#pragma unroll
+      for (int kt = 0; kt < 4; ++kt)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) wa[kt][e] = (short)w[(16 * kt + 4 * q + e) * in_dim + 16 * mt + r];
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int kt = 0; kt < 4; ++kt) {
-        bf16x4 a;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) a[e] = (short)w[(16 * kt + 4 * q + e) * in_dim + 16 * mt + r];
-        acc = mfma16(a, pack4(d[kt][0], d[kt][1], d[kt][2], d[kt][3]), acc);
-      }
+      for (int kt = 0; kt < 4; ++kt) acc = mfma16(wa[kt], bd[kt], acc);
       da[mt] = acc;
     }
   }

@@ -52,7 +52,8 @@ class SurrogateModel:
 
     def to_c(self) -> WxSurModel:
         m = WxSurModel()
-        ctypes.memmove(m.theta, self.theta.astype(np.float32).ctypes.data, NPARAM * 4)
+        th = np.ascontiguousarray(self.theta, dtype=np.float32)   # kept alive across the copy
+        ctypes.memmove(m.theta, th.ctypes.data, NPARAM * 4)
         for name in ("in_mu", "in_sig", "out_mu", "out_sig", "lo", "hi"):
             v = getattr(self, name).astype(np.float32)
             arr = getattr(m, name)
