@@ -104,8 +104,10 @@ def roofline(prof, zones, neq, workload, peak=HBM_PEAK_GBS):
             "kernels_ms": {k: round(v[0], 3) for k, v in prof.items() if v[1]}}
 
 
-def run_plain(cfg, steps, warmup):
-    """The plain FP64 path on one mesh: (zone-updates/s, ms/step, profile, final state)."""
+def run_plain(cfg, steps, warmup, events=False):
+    """The plain FP64 path on one mesh: (zone-updates/s, ms/step, kernel
+    profile, zone-updates, final state, leaf blocks).  With events, HIP events
+    bracket every launch (they add GPU work, so the throughput run has none)."""
     from walicxe2d_amd.capi import Simulation
     sim = Simulation(cfg)
     sim.basegrid()
@@ -115,12 +117,13 @@ def run_plain(cfg, steps, warmup):
             sim.admesh()
     if warmup:
         sim.step(warmup, admesh=amr)
-    sim.profile(1)                    # reset + record kernel events from here
+    if events:
+        sim.profile(1)                # reset + record kernel events from here
     sim.step(0)                       # synchronise
     t0 = time.perf_counter()
     _, _, zones = sim.step(steps, admesh=amr)   # returns after a device sync
     el = time.perf_counter() - t0
-    prof = sim.profile(0)
+    prof = sim.profile(0) if events else None
     state = _final_state(sim)
     nleaf = sim.info()[1]
     sim.close()
@@ -189,7 +192,9 @@ def main():
 
     cfg = P.CONFIGS[args.workload]()
     neq = cfg.neq
-    pv, pms, pprof, pzones, _, pleaf = run_plain(cfg, args.steps, args.warmup)
+    pv, pms, _, _, _, pleaf = run_plain(cfg, args.steps, args.warmup)
+    # the same run again with kernel events, for the roofline
+    _, _, pprof, pzones, _, _ = run_plain(cfg, args.steps, args.warmup, events=True)
     plain = {"value": pv, "ms_per_step": pms, "leaf_blocks_end": pleaf,
              "roofline": roofline(pprof, pzones, neq, args.workload),
              "step_frac": round(pv * 5 * neq * 8 / (HBM_PEAK_GBS * 1e9), 4)}

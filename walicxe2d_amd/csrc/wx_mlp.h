@@ -113,16 +113,19 @@ __device__ __forceinline__ f32x4 mlp_fwd16(const uint16_t *__restrict__ w1, cons
   return y;
 }
 
-// al = ALOSS / n^2 at (T, x_HII) from the table (bilinear in log10 T, x).
+// al = ALOSS / n^2 at (T, x_HII) from the table (bilinear in log10 T and
+// u = log10((x + 1e-4) / (1 - x + 1e-4)) on log10 al).
 __device__ __forceinline__ float sur_al(const SurDev *__restrict__ M, float lt, float x) {
   float ft = (lt - SUR_TLO) / SUR_DT;
   ft = fminf(fmaxf(ft, 0.f), (float)(SUR_NT - 1) - 1e-3f);
-  float fx = fminf(fmaxf(x, 0.f), 1.f) * (float)(SUR_NX - 1);
+  const float xc = fminf(fmaxf(x, 0.f), 1.f);
+  float fx = (log10f((xc + 1e-4f) / (1.f - xc + 1e-4f)) + 4.f) * 0.125f * (float)(SUR_NX - 1);
+  fx = fmaxf(fx, 0.f);
   fx = fminf(fx, (float)(SUR_NX - 1) - 1e-3f);
   const int it = (int)ft, ix = (int)fx;
   const float wt = ft - (float)it, wx = fx - (float)ix;
   const float *t = M->altab + it * SUR_NX + ix;
-  return (1.f - wt) * ((1.f - wx) * t[0] + wx * t[1]) + wt * ((1.f - wx) * t[SUR_NX] + wx * t[SUR_NX + 1]);
+  return exp10f((1.f - wt) * ((1.f - wx) * t[0] + wx * t[1]) + wt * ((1.f - wx) * t[SUR_NX] + wx * t[SUR_NX + 1]));
 }
 
 // Raw features of one cell (n, T, y0 = n_HI/n, dt in seconds); returns

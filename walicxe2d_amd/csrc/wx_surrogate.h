@@ -25,10 +25,14 @@ constexpr int SUR_NPARAM = 8770;
 // offsets into the FP32 parameter vector (row-major [out][in] weights)
 constexpr int SUR_W1 = 0, SUR_B1 = 256, SUR_W2 = 320, SUR_B2 = 4416, SUR_W3 = 4480, SUR_B3 = 8576,
               SUR_W4 = 8640, SUR_B4 = 8768;
-// al table: log10 T from SUR_TLO in SUR_NT steps of SUR_DT dex, x_HII in
-// SUR_NX steps over [0, 1]
-constexpr int SUR_NT = 161, SUR_NX = 21;
-constexpr float SUR_TLO = 1.0f, SUR_DT = 0.05f;
+// al table: log10(al) at log10 T from SUR_TLO in SUR_NT steps of SUR_DT
+// dex and at u = log10((x_HII + 1e-4) / (1 - x_HII + 1e-4)) in SUR_NX steps
+// over [-4, 4], interpolated bilinearly: the loss varies over many decades
+// with T, in proportion to the electron density (x_HII + 1e-4) n near
+// neutral gas and to the neutral density near full ionisation
+// (src/coolingh.f90:101-106)
+constexpr int SUR_NT = 321, SUR_NX = 81;
+constexpr float SUR_TLO = 1.0f, SUR_DT = 0.025f;
 
 struct SurDev {
   float theta[SUR_NPARAM];

@@ -35,10 +35,12 @@ __global__ __launch_bounds__(256) void k_sur_altab(SurDev *__restrict__ M) {
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < SUR_NT * SUR_NX; t += gridDim.x * blockDim.x) {
     const int it = t / SUR_NX, ix = t % SUR_NX;
     const double T = pow(10.0, (double)SUR_TLO + (double)SUR_DT * it);
-    const double y = 1.0 - (double)ix / (double)(SUR_NX - 1);
+    const double rr = pow(10.0, -4.0 + 8.0 * (double)ix / (double)(SUR_NX - 1));
+    const double x = dmin(dmax((rr * (1.0 + 1e-4) - 1e-4) / (1.0 + rr), 0.0), 1.0);
+    const double y = 1.0 - x;
     // aloss(x1 = old neutral fraction, den, dh0 = new neutral density, T) at
     // n = 1 and an unchanged fraction
-    M->altab[t] = (float)aloss(y, 1.0, y, T);
+    M->altab[t] = (float)log10(dmax(aloss(y, 1.0, y, T), 1e-300));
   }
 }
 
