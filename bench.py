@@ -244,6 +244,14 @@ def main():
                            if b["sur_cells"] else None}
         out["plain"] = plain
         out["speedup_vs_plain"] = b["value"] / pv
+        if model is not None:
+            # the same ensemble with the exact cooling: bit-identical to the
+            # single-mesh runs, so this is the batching gain alone
+            bx = run_batched(cfgs, args.steps, args.warmup, None)
+            out["batched_exact"] = {"value": bx["value"], "ms_per_step": bx["ms_per_step"],
+                                    "speedup_vs_plain": bx["value"] / pv, "persistent_grids": bx["grids"],
+                                    "threads_per_workgroup": bx["threads"],
+                                    "kernels_ms": roofline(bx["prof"], bx["prof_zones"], neq, "c3_batched")["kernels_ms"]}
     else:
         out["step_frac"] = plain["step_frac"]
     if not args.no_cpu_baseline:

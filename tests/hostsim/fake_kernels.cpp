@@ -2,6 +2,8 @@
 // statistics kernels write refinement flags from a table the test sets
 // (wxsim_set_flags), so the host's admesh can be driven with chosen flags;
 // the mesh-op launches record the ops they were given.  TEST SCAFFOLDING.
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 #include "../../walicxe2d_amd/csrc/wx_surrogate.h"
@@ -41,15 +43,28 @@ int launch_build_queue(const QueueHdr *hdr, const int32_t *inst_off, const int32
   }
   return 0;
 }
-int launch_stageA(const wx_params *, int, QueueRef, int, int, const double *, double *, DevScalars *, hipStream_t) {
-  return 0;
+// A one-entry-per-workgroup launch must cover the whole queue.
+static void check_shape(QueueRef q, int grid, int bs, int persist) {
+  if (grid <= 0 || (bs != 256 && bs != 384 && bs != 512) || (!persist && grid < q.hdr->nleaf)) {
+    fprintf(stderr, "bad stage launch: grid %d threads %d persist %d for %d queue entries\n", grid, bs, persist,
+            q.hdr->nleaf);
+    abort();
+  }
 }
-int launch_stageB(const wx_params *, int, QueueRef, int, int, const int2 *, const BlockRec *, double *, const double *,
+int launch_stageA(const wx_params *, int, QueueRef q, int grid, int bs, int persist, const double *, double *,
                   DevScalars *, hipStream_t) {
+  check_shape(q, grid, bs, persist);
   return 0;
 }
-int launch_fillI_cool(const wx_params *, int, QueueRef q, int, int, const int2 *, const BlockRec *, const double *,
-                      double *, double *stat, int8_t *flag, DevScalars *, hipStream_t) {
+int launch_stageB(const wx_params *, int, QueueRef q, int grid, int bs, int persist, const int2 *, const BlockRec *,
+                  double *, const double *, DevScalars *, hipStream_t) {
+  check_shape(q, grid, bs, persist);
+  return 0;
+}
+int launch_fillI_cool(const wx_params *, int, QueueRef q, int grid, int bs, int persist, const int2 *,
+                      const BlockRec *, const double *, double *, double *stat, int8_t *flag, DevScalars *,
+                      hipStream_t) {
+  check_shape(q, grid, bs, persist);
   write_flags(q.rec, q.hdr->nleaf, (signed char *)flag, stat);
   return 0;
 }
@@ -77,10 +92,10 @@ int launch_meshops(int, const MeshOp *ops, int nops, double *, hipStream_t) {
   return 0;
 }
 int launch_transpose(int, int, const double *, double *, int, int, hipStream_t) { return 0; }
-int launch_fillI_cool_sur(const wx_params *P, int neq, QueueRef q, int grid, int bs, const int2 *ops,
+int launch_fillI_cool_sur(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const int2 *ops,
                           const BlockRec *binfo, const double *Uin, double *Uout, double *stat, int8_t *flag,
                           DevScalars *S, SurDev *, hipStream_t st) {
-  return launch_fillI_cool(P, neq, q, grid, bs, ops, binfo, Uin, Uout, stat, flag, S, st);
+  return launch_fillI_cool(P, neq, q, grid, bs, persist, ops, binfo, Uin, Uout, stat, flag, S, st);
 }
 int sur_upload(SurDev *, const wx_sur_model *, const wx_params *, hipStream_t) { return 0; }
 int sur_counters(SurDev *, hipStream_t, int, int64_t *c, int64_t *f) {

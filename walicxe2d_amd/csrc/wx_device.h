@@ -92,12 +92,14 @@ struct MeshOp {
 int launch_build_queue(const QueueHdr *hdr, const int32_t *inst_off, const int32_t *leafbase,
                        const LeafRec *reg_rec, const int2 *reg_ops2, const int2 *reg_ops1, LeafRec *rec,
                        int2 *ops2, int2 *ops1, int grid, hipStream_t stream);
-// bs = threads per workgroup: 256, 384 or 512 (4, 6 or 8 waves)
-int launch_stageA(const wx_params *P, int neq, QueueRef q, int grid, int bs, const double *U, double *Q,
-                  DevScalars *S, hipStream_t stream);
-int launch_stageB(const wx_params *P, int neq, QueueRef q, int grid, int bs, const int2 *ops, const BlockRec *binfo,
-                  double *U, const double *Q, DevScalars *S, hipStream_t stream);
-int launch_fillI_cool(const wx_params *P, int neq, QueueRef q, int grid, int bs, const int2 *ops,
+// bs = threads per workgroup: 256, 384 or 512 (4, 6 or 8 waves).  persist
+// = 0: workgroup g takes queue entry g (grid >= queue length); 1: workgroup
+// g takes entries g, g + grid, ... (any grid).
+int launch_stageA(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const double *U,
+                  double *Q, DevScalars *S, hipStream_t stream);
+int launch_stageB(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const int2 *ops,
+                  const BlockRec *binfo, double *U, const double *Q, DevScalars *S, hipStream_t stream);
+int launch_fillI_cool(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const int2 *ops,
                       const BlockRec *binfo, const double *Uin, double *Uout, double *stat, int8_t *flag,
                       DevScalars *S, hipStream_t stream);
 // Statistics of an explicit list of leaf records (one workgroup each).

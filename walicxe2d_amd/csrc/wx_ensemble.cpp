@@ -64,9 +64,12 @@ struct wx_ens : Runtime {
   std::vector<LeafRec> dirty;
   std::unique_ptr<WorkerPool> pool;
   // launch shapes of the stage kernels (A, B, fillI): persistent workgroups
-  // (0 = one per queue entry, eager launches only) and threads per workgroup
+  // (0 = one workgroup per queue entry) and threads per workgroup
   int grid[3] = {0, 0, 0};
   int bs[3] = {256, 256, 256};
+  // queue entries the captured graphs launch workgroups for (the one-per-
+  // entry kernels): recaptured when the queue outgrows it or shrinks to half
+  int gspan = 0;
   // cooling surrogate for the coolingh instances, used when use_sur
   DevBuf<SurDev> dSur;
   bool use_sur = false;
@@ -98,11 +101,21 @@ void drop_graphs(wx_ens *e) {
   }
 }
 
-// Grid of stage kernel k for the current queue.
-int stage_grid(const wx_ens *e, int k, bool for_graph) {
-  if (e->grid[k] > 0) return e->grid[k];
-  if (for_graph) return e->ncu * (k == 2 ? 8 : 4);   // LDS-limited residency per CU
-  return e->nleaf_total;
+// Workgroups of stage kernel k for the current queue, and whether they loop
+// over it (persistent) or take one entry each.
+struct Shape {
+  int grid, persist;
+};
+Shape stage_grid(const wx_ens *e, int k, bool for_graph) {
+  if (e->grid[k] > 0) return {e->grid[k], 1};
+  return {for_graph ? e->gspan : e->nleaf_total, 0};
+}
+
+// Queue span a graph is captured for: the queue length and an eighth more,
+// so that small mesh changes replay the same graph.
+int graph_span(const wx_ens *e) {
+  const size_t want = ((size_t)e->nleaf_total * 9 / 8 + 255) / 256 * 256;
+  return (int)std::min(e->nrec, std::max<size_t>(want, 256));
 }
 
 // Recompiles the tables of instances whose mesh changed and uploads their
@@ -218,6 +231,7 @@ int launch_step(wx_ens *e, int par, bool for_graph) {
   const bool prof = !for_graph;
   double *Ucur = e->U[par].ptr, *Uoth = e->U[par ^ 1].ptr;
   const QueueRef q = e->queue();
+  const Shape sh[3] = {stage_grid(e, 0, for_graph), stage_grid(e, 1, for_graph), stage_grid(e, 2, for_graph)};
   int rc;
   {
     ProfScope ps(e, prof ? KID_DTRED : -1);
@@ -225,20 +239,20 @@ int launch_step(wx_ens *e, int par, bool for_graph) {
   }
   if (!rc) {
     ProfScope ps(e, prof ? KID_STAGEA : -1);
-    rc = launch_stageA(e->dP.ptr, e->neq, q, stage_grid(e, 0, for_graph), e->bs[0], Ucur, Uoth, e->dS.ptr, e->stream);
+    rc = launch_stageA(e->dP.ptr, e->neq, q, sh[0].grid, e->bs[0], sh[0].persist, Ucur, Uoth, e->dS.ptr, e->stream);
   }
   if (!rc) {
     ProfScope ps(e, prof ? KID_STAGEB : -1);
-    rc = launch_stageB(e->dP.ptr, e->neq, q, stage_grid(e, 1, for_graph), e->bs[1], e->dOpsII.ptr, e->dBinfo.ptr, Ucur, Uoth,
+    rc = launch_stageB(e->dP.ptr, e->neq, q, sh[1].grid, e->bs[1], sh[1].persist, e->dOpsII.ptr, e->dBinfo.ptr, Ucur, Uoth,
                        e->dS.ptr, e->stream);
   }
   if (!rc) {
     ProfScope ps(e, prof ? KID_FILLI : -1);
     if (e->use_sur)
-      rc = launch_fillI_cool_sur(e->dP.ptr, e->neq, q, stage_grid(e, 2, for_graph), e->bs[2], e->dOpsI.ptr, e->dBinfo.ptr, Ucur,
+      rc = launch_fillI_cool_sur(e->dP.ptr, e->neq, q, sh[2].grid, e->bs[2], sh[2].persist, e->dOpsI.ptr, e->dBinfo.ptr, Ucur,
                                  Uoth, e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, e->dSur.ptr, e->stream);
     else
-      rc = launch_fillI_cool(e->dP.ptr, e->neq, q, stage_grid(e, 2, for_graph), e->bs[2], e->dOpsI.ptr, e->dBinfo.ptr, Ucur,
+      rc = launch_fillI_cool(e->dP.ptr, e->neq, q, sh[2].grid, e->bs[2], sh[2].persist, e->dOpsI.ptr, e->dBinfo.ptr, Ucur,
                              Uoth, e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, e->stream);
   }
   if (!rc) {
@@ -251,8 +265,9 @@ int launch_step(wx_ens *e, int par, bool for_graph) {
 }
 
 int ensure_graphs(wx_ens *e) {
-  if (e->gexec[0] && e->gexec[1]) return WX_OK;
+  if (e->gexec[0] && e->gexec[1] && e->nleaf_total <= e->gspan && 2 * e->nleaf_total > e->gspan) return WX_OK;
   drop_graphs(e);
+  e->gspan = graph_span(e);
   for (int par = 0; par < 2; ++par) {
     HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     int rc = launch_step(e, par, true);
@@ -546,21 +561,22 @@ int wx_ens_tune(wx_ens *e, int32_t reps, int32_t *grids, int32_t *threads, doubl
     int bestg = e->nleaf_total, bestb = 256;
     for (int bsz : {256, 384, 512}) {
       for (int g : cands) {
+        const int pers = g != e->nleaf_total;
         float tot = 0.f;
         for (int r = 0; r <= reps && !rc; ++r) {   // r = 0 warms up
           if (k == 1 && hipMemcpyAsync(scratch.ptr, e->cur_u(), tiles * sizeof(double), hipMemcpyDeviceToDevice,
                                        e->stream) != hipSuccess) rc = fail(WX_EHIP, "tune copy");
           (void)hipEventRecord(a, e->stream);
           if (k == 0)
-            rc = launch_stageA(e->dP.ptr, e->neq, q, g, bsz, e->cur_u(), e->other_u(), e->dS.ptr, e->stream);
+            rc = launch_stageA(e->dP.ptr, e->neq, q, g, bsz, pers, e->cur_u(), e->other_u(), e->dS.ptr, e->stream);
           else if (k == 1)
-            rc = launch_stageB(e->dP.ptr, e->neq, q, g, bsz, e->dOpsII.ptr, e->dBinfo.ptr, scratch.ptr, e->other_u(),
+            rc = launch_stageB(e->dP.ptr, e->neq, q, g, bsz, pers, e->dOpsII.ptr, e->dBinfo.ptr, scratch.ptr, e->other_u(),
                                e->dS.ptr, e->stream);
           else if (e->use_sur)
-            rc = launch_fillI_cool_sur(e->dP.ptr, e->neq, q, g, bsz, e->dOpsI.ptr, e->dBinfo.ptr, e->cur_u(),
+            rc = launch_fillI_cool_sur(e->dP.ptr, e->neq, q, g, bsz, pers, e->dOpsI.ptr, e->dBinfo.ptr, e->cur_u(),
                                        scratch.ptr, e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, e->dSur.ptr, e->stream);
           else
-            rc = launch_fillI_cool(e->dP.ptr, e->neq, q, g, bsz, e->dOpsI.ptr, e->dBinfo.ptr, e->cur_u(), scratch.ptr,
+            rc = launch_fillI_cool(e->dP.ptr, e->neq, q, g, bsz, pers, e->dOpsI.ptr, e->dBinfo.ptr, e->cur_u(), scratch.ptr,
                                    e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, e->stream);
           (void)hipEventRecord(b, e->stream);
           (void)hipEventSynchronize(b);

@@ -124,20 +124,20 @@ int run_stages(wx_ctx *c) {
   int rc;
   {
     ProfScope ps(c, KID_STAGEA);
-    rc = launch_stageA(c->dP.ptr, c->neq, c->queue(), g, 256, c->cur_u(), c->other_u(), c->dS.ptr, c->stream);
+    rc = launch_stageA(c->dP.ptr, c->neq, c->queue(), g, 256, 0, c->cur_u(), c->other_u(), c->dS.ptr, c->stream);
   }
   if (!rc) {
     ProfScope ps(c, KID_STAGEB);
-    rc = launch_stageB(c->dP.ptr, c->neq, c->queue(), g, 256, c->dOpsII.ptr, c->dBinfo.ptr, c->cur_u(), c->other_u(),
+    rc = launch_stageB(c->dP.ptr, c->neq, c->queue(), g, 256, 0, c->dOpsII.ptr, c->dBinfo.ptr, c->cur_u(), c->other_u(),
                        c->dS.ptr, c->stream);
   }
   if (!rc) {
     ProfScope ps(c, KID_FILLI);
     if (c->use_sur)
-      rc = launch_fillI_cool_sur(c->dP.ptr, c->neq, c->queue(), g, 256, c->dOpsI.ptr, c->dBinfo.ptr, c->cur_u(),
+      rc = launch_fillI_cool_sur(c->dP.ptr, c->neq, c->queue(), g, 256, 0, c->dOpsI.ptr, c->dBinfo.ptr, c->cur_u(),
                                  c->other_u(), c->dStat.ptr, c->dFlag.ptr, c->dS.ptr, c->dSur.ptr, c->stream);
     else
-      rc = launch_fillI_cool(c->dP.ptr, c->neq, c->queue(), g, 256, c->dOpsI.ptr, c->dBinfo.ptr, c->cur_u(),
+      rc = launch_fillI_cool(c->dP.ptr, c->neq, c->queue(), g, 256, 0, c->dOpsI.ptr, c->dBinfo.ptr, c->cur_u(),
                              c->other_u(), c->dStat.ptr, c->dFlag.ptr, c->dS.ptr, c->stream);
   }
   if (rc) return fail(rc, "stage launch failed: %s", hipGetErrorString(hipGetLastError()));

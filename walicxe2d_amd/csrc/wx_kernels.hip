@@ -197,7 +197,7 @@ __device__ __forceinline__ void cyl_source(const wx_params *__restrict__ p, cons
 
 // ---------------------------------------------------------------------
 // Stage A: src/tstep.f90:22-46 (hllcfluxes choice 1 + step with dt/2).
-template <int NEQ, int BS>
+template <int NEQ, int BS, bool PERSIST>
 __global__ __launch_bounds__(BS) void k_stageA(const wx_params *__restrict__ P, QueueRef q,
                                                 const double *__restrict__ U, double *__restrict__ Q,
                                                 DevScalars *__restrict__ Sa) {
@@ -207,9 +207,11 @@ __global__ __launch_bounds__(BS) void k_stageA(const wx_params *__restrict__ P, 
   // the first entry's record is read together with the queue length
   LeafRec Lnext = q.rec[min((int)blockIdx.x, q.cap - 1)];
   const int nleaf = q.hdr->nleaf;
+  // one queue entry per workgroup, or (PERSIST) entries blockIdx.x +
+  // k * gridDim.x; the loop form costs registers, so each is its own kernel
   for (int l = blockIdx.x; l < nleaf; l += gridDim.x) {
   const LeafRec L = Lnext;
-  if (l + (int)gridDim.x < nleaf) Lnext = q.rec[l + gridDim.x];
+  if (PERSIST && l + (int)gridDim.x < nleaf) Lnext = q.rec[l + gridDim.x];
   const wx_params *__restrict__ p = P + L.inst;
   DevScalars *__restrict__ S = Sa + L.inst;
   const double *ub = U + (size_t)L.id * NEQ * NC;
@@ -243,12 +245,13 @@ __global__ __launch_bounds__(BS) void k_stageA(const wx_params *__restrict__ P, 
   }
   if (err) atomicOr(&S->err, WX_EHLLC);
   __syncthreads();   // the next queue entry reuses the LDS tiles
+   if (!PERSIST) break;   // one entry per workgroup: no loop
   }
 }
 
 // ---------------------------------------------------------------------
 // Stage B: src/tstep.f90:53-88 with boundaryII pulled from the neighbours.
-template <int NEQ, int BS>
+template <int NEQ, int BS, bool PERSIST>
 __global__ __launch_bounds__(BS) void k_stageB(const wx_params *__restrict__ P, QueueRef q,
                                                 const int2 *__restrict__ ops,
                                                 const BlockRec *__restrict__ binfo,
@@ -262,7 +265,7 @@ __global__ __launch_bounds__(BS) void k_stageB(const wx_params *__restrict__ P, 
   for (int l = blockIdx.x; l < nleaf; l += gridDim.x) {
   const int r = l;
   const LeafRec L = Lnext;
-  if (l + (int)gridDim.x < nleaf) Lnext = q.rec[l + gridDim.x];
+  if (PERSIST && l + (int)gridDim.x < nleaf) Lnext = q.rec[l + gridDim.x];
   const wx_params *__restrict__ p = P + L.inst;
   DevScalars *__restrict__ S = Sa + L.inst;
   const int b = L.id;
@@ -333,6 +336,7 @@ __global__ __launch_bounds__(BS) void k_stageB(const wx_params *__restrict__ P, 
   }
   if (err) atomicOr(&S->err, WX_EHLLC);
   __syncthreads();   // the next queue entry reuses the LDS tiles
+   if (!PERSIST) break;   // one entry per workgroup: no loop
   }
 }
 
@@ -404,8 +408,41 @@ __device__ void block_stats(const wx_params *__restrict__ p, const double *prim,
 // (src/coolingh.f90:249-250) evaluated with the tabulated loss; the energy
 // residual is the change of thermal energy (2 - y1) T1 that correction
 // makes, relative to the old thermal energy (2 - y0) T.
+// The network's weights and biases staged in LDS once per workgroup.
+struct SurLds {
+  uint16_t w1[64 * 16], w2[64 * 64], w3[64 * 64], w4[16 * 64];
+  float b[64 * 3 + 2];
+};
+
+template <bool SUR>
+struct SurSlot {
+  using type = SurLds;
+};
+template <>
+struct SurSlot<false> {
+  using type = char;
+};
+
+__device__ void sur_stage(const SurDev *__restrict__ M, SurLds *L) {
+  for (int t = threadIdx.x; t < 64 * 64 / 4; t += blockDim.x) {   // 8-byte chunks
+    reinterpret_cast<uint2 *>(L->w2)[t] = reinterpret_cast<const uint2 *>(M->w2)[t];
+    reinterpret_cast<uint2 *>(L->w3)[t] = reinterpret_cast<const uint2 *>(M->w3)[t];
+    if (t < 64 * 16 / 4) {
+      reinterpret_cast<uint2 *>(L->w1)[t] = reinterpret_cast<const uint2 *>(M->w1)[t];
+      reinterpret_cast<uint2 *>(L->w4)[t] = reinterpret_cast<const uint2 *>(M->w4)[t];
+    }
+  }
+  for (int t = threadIdx.x; t < 64; t += blockDim.x) {
+    L->b[t] = M->theta[SUR_B1 + t];
+    L->b[64 + t] = M->theta[SUR_B2 + t];
+    L->b[128 + t] = M->theta[SUR_B3 + t];
+  }
+  if (threadIdx.x < 2) L->b[192 + threadIdx.x] = M->theta[SUR_B4 + threadIdx.x];
+}
+
 template <int NEQ>
-__device__ void cool_surrogate(const wx_params *__restrict__ p, SurDev *__restrict__ M, double dts, double *tile) {
+__device__ void cool_surrogate(const wx_params *__restrict__ p, SurDev *__restrict__ M, const SurLds *W, double dts,
+                               double *tile) {
   __shared__ float sx[324 * 4];
   __shared__ float sy[324 * 2];
   __shared__ float scedt[324];
@@ -432,7 +469,7 @@ __device__ void cool_surrogate(const wx_params *__restrict__ p, SurDev *__restri
       const int c = 16 * g + r;
       bf16x4 xb = {0, 0, 0, 0};
       if (qq == 0 && c < 324) xb = pack4(sx[4 * c], sx[4 * c + 1], sx[4 * c + 2], sx[4 * c + 3]);
-      const f32x4 y = mlp_fwd16(M->w1, M->w2, M->w3, M->w4, M->theta, xb);
+      const f32x4 y = mlp_fwd16(W->w1, W->w2, W->w3, W->w4, W->b, W->b + 64, W->b + 128, W->b + 192, xb);
       if (qq == 0 && c < 324) {
         sy[2 * c] = y[0];
         sy[2 * c + 1] = y[1];
@@ -482,7 +519,7 @@ __device__ void cool_surrogate(const wx_params *__restrict__ p, SurDev *__restri
   }
 }
 
-template <int NEQ, bool SUR, int BS>
+template <int NEQ, bool SUR, int BS, bool PERSIST>
 __global__ __launch_bounds__(BS) void k_fillI_cool(const wx_params *__restrict__ P, QueueRef q,
                                                     const int2 *__restrict__ ops,
                                                     const BlockRec *__restrict__ binfo,
@@ -493,12 +530,20 @@ __global__ __launch_bounds__(BS) void k_fillI_cool(const wx_params *__restrict__
                                                     DevScalars *__restrict__ Sa,
                                                     SurDev *__restrict__ M) {
   __shared__ double tile[NEQ * NC];
+  // the network in LDS, staged once per workgroup (no LDS without the surrogate)
+  __shared__ __align__(16) typename SurSlot<SUR>::type Wbuf;
+  SurLds *W = nullptr;
+  if constexpr (SUR) {
+    W = &Wbuf;
+    sur_stage(M, W);
+    __syncthreads();
+  }
   LeafRec Lnext = q.rec[min((int)blockIdx.x, q.cap - 1)];
   const int nleaf = q.hdr->nleaf;
   for (int l = blockIdx.x; l < nleaf; l += gridDim.x) {
   const int r = l;
   const LeafRec L = Lnext;
-  if (l + (int)gridDim.x < nleaf) Lnext = q.rec[l + gridDim.x];
+  if (PERSIST && l + (int)gridDim.x < nleaf) Lnext = q.rec[l + gridDim.x];
   const wx_params *__restrict__ p = P + L.inst;
   DevScalars *__restrict__ S = Sa + L.inst;
   const int b = L.id;
@@ -539,7 +584,7 @@ __global__ __launch_bounds__(BS) void k_fillI_cool(const wx_params *__restrict__
   bool done = false;
   if constexpr (SUR && NEQ >= 5) {
     if (mode == 1) {
-      cool_surrogate<NEQ>(p, M, S->dt * p->tsc, tile);
+      cool_surrogate<NEQ>(p, M, W, S->dt * p->tsc, tile);
       __syncthreads();
       done = true;
     }
@@ -574,6 +619,7 @@ __global__ __launch_bounds__(BS) void k_fillI_cool(const wx_params *__restrict__
   __syncthreads();
   block_stats<NEQ>(p, tile, L, stat, flag);
   __syncthreads();   // the next queue entry reuses the LDS tile
+   if (!PERSIST) break;   // one entry per workgroup: no loop
   }
 }
 
@@ -798,7 +844,8 @@ int launch_build_queue(const QueueHdr *hdr, const int32_t *inst_off, const int32
   return hip_status();
 }
 
-// Workgroup sizes the stage kernels are built for (4, 6 or 8 waves).
+// Workgroup sizes the stage kernels are built for (4, 6 or 8 waves), and
+// the one-entry-per-workgroup / persistent forms.
 #define WX_BS_DISPATCH(BSV, CALL)                    \
   switch (BSV) {                                     \
     case 256: { constexpr int B = 256; CALL; } break; \
@@ -806,36 +853,39 @@ int launch_build_queue(const QueueHdr *hdr, const int32_t *inst_off, const int32
     case 512: { constexpr int B = 512; CALL; } break; \
     default: return WX_EARG;                         \
   }
+#define WX_PERSIST_DISPATCH(PV, CALL)                     \
+  if (PV) { constexpr bool PS = true; CALL; }             \
+  else { constexpr bool PS = false; CALL; }
 
-int launch_stageA(const wx_params *P, int neq, QueueRef q, int grid, int bs, const double *U, double *Q,
-                  DevScalars *S, hipStream_t stream) {
+int launch_stageA(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const double *U,
+                  double *Q, DevScalars *S, hipStream_t stream) {
   if (grid <= 0) return WX_OK;
-  WX_BS_DISPATCH(bs, WX_DISPATCH(neq, hipLaunchKernelGGL((k_stageA<N, B>), dim3(grid), dim3(B), 0, stream, P, q, U, Q, S)));
+  WX_PERSIST_DISPATCH(persist, WX_BS_DISPATCH(bs, WX_DISPATCH(neq, hipLaunchKernelGGL((k_stageA<N, B, PS>), dim3(grid), dim3(B), 0, stream, P, q, U, Q, S))));
   return hip_status();
 }
 
-int launch_stageB(const wx_params *P, int neq, QueueRef q, int grid, int bs, const int2 *ops, const BlockRec *binfo,
-                  double *U, const double *Q, DevScalars *S, hipStream_t stream) {
+int launch_stageB(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const int2 *ops,
+                  const BlockRec *binfo, double *U, const double *Q, DevScalars *S, hipStream_t stream) {
   if (grid <= 0) return WX_OK;
-  WX_BS_DISPATCH(bs, WX_DISPATCH(neq, hipLaunchKernelGGL((k_stageB<N, B>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, U, Q, S)));
+  WX_PERSIST_DISPATCH(persist, WX_BS_DISPATCH(bs, WX_DISPATCH(neq, hipLaunchKernelGGL((k_stageB<N, B, PS>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, U, Q, S))));
   return hip_status();
 }
 
-int launch_fillI_cool(const wx_params *P, int neq, QueueRef q, int grid, int bs, const int2 *ops,
+int launch_fillI_cool(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const int2 *ops,
                       const BlockRec *binfo, const double *Uin, double *Uout, double *stat, int8_t *flag,
                       DevScalars *S, hipStream_t stream) {
   if (grid <= 0) return WX_OK;
-  WX_BS_DISPATCH(bs, WX_DISPATCH(neq, hipLaunchKernelGGL((k_fillI_cool<N, false, B>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, nullptr)));
+  WX_PERSIST_DISPATCH(persist, WX_BS_DISPATCH(bs, WX_DISPATCH(neq, hipLaunchKernelGGL((k_fillI_cool<N, false, B, PS>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, nullptr))));
   return hip_status();
 }
 
-int launch_fillI_cool_sur(const wx_params *P, int neq, QueueRef q, int grid, int bs, const int2 *ops,
+int launch_fillI_cool_sur(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const int2 *ops,
                           const BlockRec *binfo, const double *Uin, double *Uout, double *stat, int8_t *flag,
                           DevScalars *S, SurDev *M, hipStream_t stream) {
   if (grid <= 0) return WX_OK;
   switch (neq) {   // coolingh needs u(5) = n_HI
-    case 5: WX_BS_DISPATCH(bs, hipLaunchKernelGGL((k_fillI_cool<5, true, B>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, M)); break;
-    case 6: WX_BS_DISPATCH(bs, hipLaunchKernelGGL((k_fillI_cool<6, true, B>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, M)); break;
+    case 5: WX_PERSIST_DISPATCH(persist, WX_BS_DISPATCH(bs, hipLaunchKernelGGL((k_fillI_cool<5, true, B, PS>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, M))); break;
+    case 6: WX_PERSIST_DISPATCH(persist, WX_BS_DISPATCH(bs, hipLaunchKernelGGL((k_fillI_cool<6, true, B, PS>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, M))); break;
     default: return WX_EARG;
   }
   return hip_status();

@@ -50,24 +50,26 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
 }
 
 
-__device__ __forceinline__ float silu(float z) { return __fdividef(z, 1.f + __expf(-z)); }
-
+// SiLU and its derivative with the hardware exp2 and reciprocal (1 ulp each).
+__device__ __forceinline__ float sigm(float z) { return __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
 __device__ __forceinline__ float dsilu(float z) {
-  const float s = __fdividef(1.f, 1.f + __expf(-z));
+  const float s = sigm(z);
   return s * (1.f + z * (1.f - s));
 }
 
 // Forward pass of 16 samples by one wave.  x: B operand of layer 1 (lanes
 // 0-15 carry their sample's 4 standardised features, lanes 16-63 zeros).
-// w*: bf16 weights as in SurDev; b: the FP32 parameter vector (biases read
-// at the SUR_B* offsets).  Returns the D tile of layer 4 (rows 0, 1 =
-// standardised outputs of sample l in lanes l < 16).  If zs / as are given,
-// the pre-activations (D layout) and bf16 activations are kept for the
-// backward pass.
+// w*: bf16 weights as in SurDev; b1..b4: the FP32 biases (global memory or
+// LDS copies).  Returns the D tile of layer 4 (rows 0, 1 = standardised
+// outputs of sample l in lanes l < 16).  If zs / as are given, the
+// pre-activations (D layout) and bf16 activations are kept for the backward
+// pass.
 __device__ __forceinline__ f32x4 mlp_fwd16(const uint16_t *__restrict__ w1, const uint16_t *__restrict__ w2,
                                            const uint16_t *__restrict__ w3, const uint16_t *__restrict__ w4,
-                                           const float *__restrict__ th, bf16x4 x, f32x4 (*zs)[4] = nullptr,
-                                           bf16x4 (*as)[4] = nullptr) {
+                                           const float *__restrict__ b1, const float *__restrict__ b2,
+                                           const float *__restrict__ b3, const float *__restrict__ b4, bf16x4 x,
+                                           f32x4 (*zs)[4] = nullptr, bf16x4 (*as)[4] = nullptr) {
   const int l = threadIdx.x & 63;
   const int r = l & 15, q = l >> 4;
   bf16x4 a[4];
@@ -77,7 +79,7 @@ __device__ __forceinline__ f32x4 mlp_fwd16(const uint16_t *__restrict__ w1, cons
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     acc = mfma16(ld4(w1 + (16 * mt + r) * 16 + 4 * q), x, acc);
 #pragma unroll
-    for (int e = 0; e < 4; ++e) acc[e] += th[SUR_B1 + 16 * mt + 4 * q + e];
+    for (int e = 0; e < 4; ++e) acc[e] += b1[16 * mt + 4 * q + e];
     if (zs) zs[0][mt] = acc;
     a[mt] = pack4(silu(acc[0]), silu(acc[1]), silu(acc[2]), silu(acc[3]));
     if (as) as[0][mt] = a[mt];
@@ -86,7 +88,7 @@ __device__ __forceinline__ f32x4 mlp_fwd16(const uint16_t *__restrict__ w1, cons
 #pragma unroll
   for (int layer = 0; layer < 2; ++layer) {
     const uint16_t *__restrict__ w = layer == 0 ? w2 : w3;
-    const int boff = layer == 0 ? SUR_B2 : SUR_B3;
+    const float *__restrict__ b = layer == 0 ? b2 : b3;
     bf16x4 an[4];
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
@@ -94,7 +96,7 @@ __device__ __forceinline__ f32x4 mlp_fwd16(const uint16_t *__restrict__ w1, cons
 #pragma unroll
       for (int kt = 0; kt < 4; ++kt) acc = mfma16(ld4(w + (16 * mt + r) * 64 + 16 * kt + 4 * q), a[kt], acc);
 #pragma unroll
-      for (int e = 0; e < 4; ++e) acc[e] += th[boff + 16 * mt + 4 * q + e];
+      for (int e = 0; e < 4; ++e) acc[e] += b[16 * mt + 4 * q + e];
       if (zs) zs[layer + 1][mt] = acc;
       an[mt] = pack4(silu(acc[0]), silu(acc[1]), silu(acc[2]), silu(acc[3]));
       if (as) as[layer + 1][mt] = an[mt];
@@ -107,10 +109,17 @@ __device__ __forceinline__ f32x4 mlp_fwd16(const uint16_t *__restrict__ w1, cons
 #pragma unroll
   for (int kt = 0; kt < 4; ++kt) y = mfma16(ld4(w4 + r * 64 + 16 * kt + 4 * q), a[kt], y);
   if (q == 0) {
-    y[0] += th[SUR_B4];
-    y[1] += th[SUR_B4 + 1];
+    y[0] += b4[0];
+    y[1] += b4[1];
   }
   return y;
+}
+
+// The forward pass with the weights and biases of a model in global memory.
+__device__ __forceinline__ f32x4 mlp_fwd16(const SurDev *__restrict__ M, bf16x4 x, f32x4 (*zs)[4] = nullptr,
+                                           bf16x4 (*as)[4] = nullptr) {
+  return mlp_fwd16(M->w1, M->w2, M->w3, M->w4, M->theta + SUR_B1, M->theta + SUR_B2, M->theta + SUR_B3,
+                   M->theta + SUR_B4, x, zs, as);
 }
 
 // al = ALOSS / n^2 at (T, x_HII) from the table (bilinear in log10 T and

@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void k_sur_infer(const SurDev *__restrict__ M,
     const int64_t s = 16 * g + r;
     bf16x4 xb = {0, 0, 0, 0};
     if (q == 0 && s < n) xb = pack4(x[4 * s], x[4 * s + 1], x[4 * s + 2], x[4 * s + 3]);
-    const f32x4 yt = mlp_fwd16(M->w1, M->w2, M->w3, M->w4, M->theta, xb);
+    const f32x4 yt = mlp_fwd16(M, xb);
     if (q == 0 && s < n) {
       y[2 * s] = yt[0];
       y[2 * s + 1] = yt[1];
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(256) void k_sur_grad(const SurDev *__restrict__ M, 
     if (q == 0 && valid) xb = pack4(x[4 * s], x[4 * s + 1], x[4 * s + 2], x[4 * s + 3]);
     f32x4 zs[3][4];
     bf16x4 as[3][4];
-    const f32x4 y = mlp_fwd16(M->w1, M->w2, M->w3, M->w4, M->theta, xb, zs, as);
+    const f32x4 y = mlp_fwd16(M, xb, zs, as);
 #pragma unroll
     for (int e = 0; e < 4; ++e) act[w][0][(4 * q + e) * 16 + r] = (uint16_t)xb[e];
 #pragma unroll
