@@ -1,8 +1,9 @@
 #!/usr/bin/env python3
 """Throughput of the hot path on one MI355X, in zone-updates/s: the plain
 FP64 path on one mesh, and the batched path (many jet variants in one
-ensemble, cooling surrogate, autotuned persistent launches in hipGraphs)
-at a fixed relative L1 density error against the plain FP64 path.
+ensemble over a compacted block queue, autotuned launches in hipGraphs),
+with the exact cooling and with the cooling surrogate, the latter held to a
+relative L1 density error of 1e-3 against the exact run of each variant.
 
 A step is one iteration of the reference's time loop (src/main.f90:87-98):
 CFL timestep, time += dt, tstep (stage A, ghost fill II, stage B + viscosity,
@@ -11,17 +12,18 @@ stays resident in HBM.  A zone-update is one interior leaf cell advanced by
 one step (ghost cells not counted).  The timed region is K steps after W
 warm-up steps, bracketed by device synchronisation.
 
-Prints one JSON line: `value` is the batched path's throughput (the plain
-path's with --path plain), `plain` holds the plain path's figures,
-`accuracy` the L1 density error of the batched path's first instances
-against plain FP64 runs of the same variants.  Options:
+Prints one JSON line: `value` is the batched path's throughput -- the
+surrogate run's when it holds the L1 bound and is faster, else the exact
+run's (the plain path's with --path plain); `plain`, `batched` and
+`surrogate` hold each run's figures.  Options:
   --gpus N          only N = 1 is supported (one GPU holds every leaf block)
   --steps K --warmup W   default K=12000, W=50: the cooling jet from t=0 to
                          about 300 yr (the configured end time), i.e. the
                          whole run with its growing AMR mesh
   --workload NAME   c3 (default: the variable cooling jet), c2, c1
   --batch B         jet variants in the ensemble (default 16)
-  --path plain      the plain path only
+  --path plain|batched   one path only (default: both)
+  --no-surrogate    skip the batched+surrogate run
   --no-cpu-baseline skip the CPU oracle timing
 """
 from __future__ import annotations
@@ -37,6 +39,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "zone-updates/s, 1 MI355X: plain FP64 vs batched+surrogate at fixed L1 error"
+L1_BOUND = 1e-3   # relative L1 density error the batched path is held to
 
 # Algorithmic HBM bytes per zone-update of each kernel (neq doubles per cell):
 #   stage A      read U^n, write U^{n+1/2}                       2*neq*8
@@ -136,10 +139,10 @@ def _final_state(sim):
     return u, lp, leaf, ic
 
 
-def run_batched(cfgs, steps, warmup, model):
+def run_batched(cfgs, steps, warmup, model, nstates=2):
     """The batched path: every instance in one ensemble, surrogate cooling
-    (when a model is given), autotuned persistent launches captured in
-    hipGraphs."""
+    (when a model is given), autotuned launch shapes, steps replayed from
+    hipGraphs.  Returns the final states of the first `nstates` instances."""
     from walicxe2d_amd.capi import Ensemble
     ens = Ensemble(cfgs)
     ens.basegrid()
@@ -162,7 +165,7 @@ def run_batched(cfgs, steps, warmup, model):
     pz = ens.step(min(steps, 200))
     prof = ens.profile(0)
     states = []
-    for i in range(min(2, len(cfgs))):
+    for i in range(min(nstates, len(cfgs))):
         u = ens.download(i)
         lp, leaf, ic, _, _ = ens.tree(i)
         states.append((u, lp, leaf, ic))
@@ -180,7 +183,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="c3")
     ap.add_argument("--batch", type=int, default=16, help="jet variants in the batched ensemble")
-    ap.add_argument("--path", choices=["both", "plain"], default="both")
+    ap.add_argument("--path", choices=["both", "plain", "batched"], default="both")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the batched+surrogate run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -192,12 +196,15 @@ def main():
 
     cfg = P.CONFIGS[args.workload]()
     neq = cfg.neq
-    pv, pms, _, _, _, pleaf = run_plain(cfg, args.steps, args.warmup)
-    # the same run again with kernel events, for the roofline
-    _, _, pprof, pzones, _, _ = run_plain(cfg, args.steps, args.warmup, events=True)
-    plain = {"value": pv, "ms_per_step": pms, "leaf_blocks_end": pleaf,
-             "roofline": roofline(pprof, pzones, neq, args.workload),
-             "step_frac": round(pv * 5 * neq * 8 / (HBM_PEAK_GBS * 1e9), 4)}
+    plain = None
+    pv, pms, pleaf, proof = None, None, None, None
+    if args.path in ("both", "plain"):
+        pv, pms, _, _, _, pleaf = run_plain(cfg, args.steps, args.warmup)
+        # the same run again with kernel events, for the roofline
+        _, _, pprof, pzones, _, _ = run_plain(cfg, args.steps, args.warmup, events=True)
+        proof = roofline(pprof, pzones, neq, args.workload)
+        plain = {"value": pv, "ms_per_step": pms, "leaf_blocks_end": pleaf, "roofline": proof,
+                 "step_frac": round(pv * 5 * neq * 8 / (HBM_PEAK_GBS * 1e9), 4)}
     out = {
         "metric": METRIC,
         "value": pv,
@@ -214,45 +221,51 @@ def main():
         "config": {"workload": cfg.name, "path": "plain-fp64", "roots": [cfg.nbrootx, cfg.nbrooty],
                    "nlevs": cfg.nlevs, "neq": neq, "cooling": cfg.cooling, "leaf_blocks_end": pleaf,
                    "block": "16x16+2 ghosts", "admesh_every_step": cfg.nlevs > 1},
-        "roofline": plain["roofline"],
+        "roofline": proof,
     }
-    if args.path == "both" and args.workload == "c3":
+    if args.path in ("both", "batched") and args.workload == "c3":
         from walicxe2d_amd import surrogate as S
         from walicxe2d_amd.l1 import l1_density
-        model = S.SurrogateModel.load() if os.path.exists(S.MODEL_PATH) else None
         cfgs = [P.config_c5(100 + k) for k in range(args.batch)]
-        b = run_batched(cfgs, args.steps, args.warmup, model)
-        # accuracy: the first instances against plain FP64 runs of the same
-        # variants over the same steps (the plain path equals the oracle to
-        # 1e-12 per cell per step)
-        l1 = []
-        for i, st in enumerate(b["states"]):
-            _, _, _, _, ref, _ = run_plain(cfgs[i], args.steps, args.warmup)
-            l1.append(l1_density(cfgs[i], st, ref))
-        out["value"] = b["value"]
-        out["ms_per_step"] = b["ms_per_step"]
-        out["dtype"] = "f64 hydro, bf16/fp32 cooling surrogate" if model is not None else "f64"
-        out["config"].update({"path": "batched+surrogate" if model is not None else "batched",
-                              "instances": args.batch, "variants": "jet seeds 777+100..777+%d" % (99 + args.batch),
-                              "leaf_blocks_end_total": b["nleaf"], "persistent_grids": b["grids"],
-                              "threads_per_workgroup": b["threads"],
-                              "graph": True})
-        out["roofline"] = roofline(b["prof"], b["prof_zones"], neq, "c3_batched")
-        out["accuracy"] = {"l1_density_rel_vs_fp64": max(l1) if l1 else None, "instances_checked": len(l1),
-                           "bound": 1e-3,
-                           "surrogate_fallback_fraction": (b["sur_fallbacks"] / b["sur_cells"])
-                           if b["sur_cells"] else None}
-        out["plain"] = plain
-        out["speedup_vs_plain"] = b["value"] / pv
-        if model is not None:
-            # the same ensemble with the exact cooling: bit-identical to the
-            # single-mesh runs, so this is the batching gain alone
-            bx = run_batched(cfgs, args.steps, args.warmup, None)
-            out["batched_exact"] = {"value": bx["value"], "ms_per_step": bx["ms_per_step"],
-                                    "speedup_vs_plain": bx["value"] / pv, "persistent_grids": bx["grids"],
-                                    "threads_per_workgroup": bx["threads"],
-                                    "kernels_ms": roofline(bx["prof"], bx["prof_zones"], neq, "c3_batched")["kernels_ms"]}
-    else:
+        # the batched path with the exact cooling: bit-identical to single-mesh
+        # runs of the same variants (so its L1 error is zero by construction)
+        bx = run_batched(cfgs, args.steps, args.warmup, None, nstates=args.batch)
+        batched = {"value": bx["value"], "ms_per_step": bx["ms_per_step"],
+                   "leaf_blocks_end_total": bx["nleaf"], "persistent_grids": bx["grids"],
+                   "threads_per_workgroup": bx["threads"],
+                   "roofline": roofline(bx["prof"], bx["prof_zones"], neq, "c3_batched")}
+        best, best_path, best_prof = bx, "batched", batched["roofline"]
+        model = S.SurrogateModel.load() if os.path.exists(S.MODEL_PATH) else None
+        sur = None
+        if model is not None and not args.no_surrogate:
+            bs = run_batched(cfgs, args.steps, args.warmup, model, nstates=args.batch)
+            # L1 density error of every instance against its exact run
+            l1 = [l1_density(cfgs[i], bs["states"][i], bx["states"][i]) for i in range(len(cfgs))]
+            sur = {"value": bs["value"], "ms_per_step": bs["ms_per_step"],
+                   "l1_density_rel_vs_fp64_max": max(l1), "l1_density_rel_vs_fp64_median": float(sorted(l1)[len(l1) // 2]),
+                   "instances_checked": len(l1), "bound": L1_BOUND,
+                   "surrogate_fallback_fraction": (bs["sur_fallbacks"] / bs["sur_cells"]) if bs["sur_cells"] else None,
+                   "kernels_ms": roofline(bs["prof"], bs["prof_zones"], neq, "c3_batched")["kernels_ms"]}
+            # the surrogate path counts only where it holds the L1 bound
+            if max(l1) <= L1_BOUND and bs["value"] > bx["value"]:
+                best, best_path = bs, "batched+surrogate"
+                best_prof = roofline(bs["prof"], bs["prof_zones"], neq, "c3_batched")
+        out["value"] = best["value"]
+        out["ms_per_step"] = best["ms_per_step"]
+        out["dtype"] = "f64 hydro, bf16/fp32 cooling surrogate" if best_path.endswith("surrogate") else "f64"
+        out["config"].update({"path": best_path, "instances": args.batch,
+                              "variants": "jet seeds 777+100..777+%d" % (99 + args.batch),
+                              "leaf_blocks_end_total": best["nleaf"], "persistent_grids": best["grids"],
+                              "threads_per_workgroup": best["threads"], "graph": True})
+        out["roofline"] = best_prof
+        out["batched"] = batched
+        out["surrogate"] = sur
+        out["accuracy"] = {"l1_density_rel_vs_fp64": 0.0 if best_path == "batched" else sur["l1_density_rel_vs_fp64_max"],
+                           "bound": L1_BOUND}
+        if plain is not None:
+            out["plain"] = plain
+            out["speedup_vs_plain"] = best["value"] / plain["value"]
+    elif plain is not None:
         out["step_frac"] = plain["step_frac"]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(P.CONFIGS[args.workload]())

@@ -520,7 +520,7 @@ __device__ void cool_surrogate(const wx_params *__restrict__ p, SurDev *__restri
 }
 
 template <int NEQ, bool SUR, int BS, bool PERSIST>
-__global__ __launch_bounds__(BS) void k_fillI_cool(const wx_params *__restrict__ P, QueueRef q,
+__global__ __launch_bounds__(BS, (SUR ? 2 : 1024 / BS)) void k_fillI_cool(const wx_params *__restrict__ P, QueueRef q,
                                                     const int2 *__restrict__ ops,
                                                     const BlockRec *__restrict__ binfo,
                                                     const double *__restrict__ Uin,

@@ -164,8 +164,16 @@ __device__ __forceinline__ int riemann(const wx_params *__restrict__ p, const do
   double csr = sqrt(gam * pr[3] / pr[0]);
   double sr = dmax(pl[1] + csl, pr[1] + csr);
   double sl = dmin(pl[1] - csl, pr[1] - csr);
-  if (sl > 0) { primf<NEQ>(cv, pl, ff); return 0; }
-  if (sr < 0) { primf<NEQ>(cv, pr, ff); return 0; }
+  // supersonic faces take the upwind state's flux; the side is selected
+  // first so that a wave evaluates primf once whichever sides its lanes need
+  if (sl > 0 || sr < 0) {
+    const bool up = sl > 0;
+    double qu[NEQ];
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) qu[k] = up ? pl[k] : pr[k];
+    primf<NEQ>(cv, qu, ff);
+    return 0;
+  }
   if (solver == 1) {
     double fl[NEQ], fr[NEQ], ul[NEQ], ur[NEQ];
     primf<NEQ>(cv, pl, fl); primf<NEQ>(cv, pr, fr);
@@ -180,11 +188,17 @@ __device__ __forceinline__ int riemann(const wx_params *__restrict__ p, const do
   double rholul = pl[0] * pl[1];
   double rhorur = pr[0] * pr[1];
   double sst = (srmur * rhorur - slmul * rholul - pr[3] + pl[3]) / (srmur * pr[0] - slmul * pl[0]);
-  int err = 0;
-  if (sst >= 0.0) hllc_star<NEQ>(p, pl, sl, slmul, sst, false, ff);
-  else if (sst <= 0.0) hllc_star<NEQ>(p, pr, sr, srmur, sst, true, ff);
-  else {
-    err = 1;
+  // star region of the side sst points away from (one evaluation per lane,
+  // the side selected first); a NaN star speed is the reference's `stop`
+  const bool left = sst >= 0.0;
+  const int err = (left || sst <= 0.0) ? 0 : 1;
+  {
+    double qs[NEQ];
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) qs[k] = left ? pl[k] : pr[k];
+    hllc_star<NEQ>(p, qs, left ? sl : sr, left ? slmul : srmur, sst, !left, ff);
+  }
+  if (err) {
 #pragma unroll
     for (int k = 0; k < NEQ; ++k) ff[k] = 0.0;
   }
@@ -233,19 +247,6 @@ __device__ __forceinline__ double aloss(double x1, double den, double dh0, doubl
   ECOLL = dmax(ECOLL, 0.);
   double CION = 5.834E-11 * sqrt(TE) * exp(-(1.579E5 / TE));
   double EION = DE * dh0 * CION * XION;
-  double EREC = DE * DHP * (rate_betah(TE));
-  EREC = dmax(EREC, 0.);
-  double TM = 1. / TE;
-  double T2 = TM * TM;
-  // 10.**x and TE**0.5 of the routine as exp10 and sqrt (within an ulp of
-  // the power function, at a fraction of its cost)
-  double EOI = DE * DOI * exp10(1381465 * T2 - 12328.69 * TM - 19.82621);
-  double EOII = DE * DOII * exp10(-(2061075. * T2) - 14596.24 * TM - 19.01402);
-  EOI = dmax(EOI, 0.);
-  EOII = dmax(EOII, 0.);
-  double BETAF = 1.3 * 1.42E-27 * sqrt(TE);
-  double HIICOOL = DE * DHP * BETAF;
-  double EQUIL = (1.0455E-18 / pow(TE, 0.63)) * (1. - exp(-pow(TE * 1.E-5, 1.63))) * DE * den + HIICOOL;
   double FR = 0.0;
   if (TE <= 44770.) FR = 0.;
   if (TE >= 54770.) FR = 1.;
@@ -254,7 +255,31 @@ __device__ __forceinline__ double aloss(double x1, double den, double dh0, doubl
     double TANH = (1. - EX2) / (1. + EX2);
     FR = 0.5 * (1. + TANH);
   }
-  return ECOLL + EION + (EREC + 7.033 * (EOI + EOII)) * (1. - FR) + EQUIL * FR;
+  // The recombination + oxygen terms enter with weight 1 - FR and the
+  // equilibrium curve with weight FR: each is evaluated only where its weight
+  // is nonzero (both are finite, so a skipped term contributes the same +0
+  // as the routine's product with a zero weight).
+  double LOW = 0.0, HIGH = 0.0;
+  if (TE < 54770.) {
+    double EREC = DE * DHP * (rate_betah(TE));
+    EREC = dmax(EREC, 0.);
+    double TM = 1. / TE;
+    double T2 = TM * TM;
+    // 10.**x of the routine as exp10 (within an ulp of the power function,
+    // at a fraction of its cost)
+    double EOI = DE * DOI * exp10(1381465 * T2 - 12328.69 * TM - 19.82621);
+    double EOII = DE * DOII * exp10(-(2061075. * T2) - 14596.24 * TM - 19.01402);
+    EOI = dmax(EOI, 0.);
+    EOII = dmax(EOII, 0.);
+    LOW = EREC + 7.033 * (EOI + EOII);
+  }
+  if (TE > 44770.) {
+    // TE**0.5 as sqrt
+    double BETAF = 1.3 * 1.42E-27 * sqrt(TE);
+    double HIICOOL = DE * DHP * BETAF;
+    HIGH = (1.0455E-18 / pow(TE, 0.63)) * (1. - exp(-pow(TE * 1.E-5, 1.63))) * DE * den + HIICOOL;
+  }
+  return ECOLL + EION + LOW * (1. - FR) + HIGH * FR;
 }
 
 // src/coolingh.f90:171-259 atomic: H neutral fraction + radiative loss.
