@@ -128,52 +128,46 @@ __device__ __forceinline__ int flux_sweep(const wx_params *__restrict__ p, const
   int err = 0;
   const int lim = p->limiter;
   for (int f = threadIdx.x; f < 544; f += blockDim.x) {
+    // x-face fi on row j, or y-face fj on column i: the cells across the face
+    // are `stride` apart, and a y-face's states are swapped into the x frame
+    // (swapy, src/swapy.f90:4-13), so one solver call serves both
+    const bool yf = f >= 272;
+    int c0, stride;
+    if (!yf) {
+      c0 = IDX(f % 17, f / 17 + 1);
+      stride = 1;
+    } else {
+      const int g = f - 272;
+      c0 = IDX(g % 16 + 1, g / 16);
+      stride = NP;
+    }
     double ql[NEQ], qr[NEQ], ff[NEQ];
-    if (f < 272) {
-      const int fi = f % 17, j = f / 17 + 1;
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) {
+      const int kk = (k == 1 || k == 2) && yf ? 3 - k : k;
+      ql[k] = prim[kk * NC + c0];
+      qr[k] = prim[kk * NC + c0 + stride];
+    }
+    if (CHOICE == 2) {
+      double qll[NEQ], qrr[NEQ];
 #pragma unroll
       for (int k = 0; k < NEQ; ++k) {
-        ql[k] = prim[k * NC + IDX(fi, j)];
-        qr[k] = prim[k * NC + IDX(fi + 1, j)];
+        const int kk = (k == 1 || k == 2) && yf ? 3 - k : k;
+        qll[k] = prim[kk * NC + c0 - stride];
+        qrr[k] = prim[kk * NC + c0 + 2 * stride];
       }
-      if (CHOICE == 2) {
-        double qll[NEQ], qrr[NEQ];
-#pragma unroll
-        for (int k = 0; k < NEQ; ++k) {
-          qll[k] = prim[k * NC + IDX(fi - 1, j)];
-          qrr[k] = prim[k * NC + IDX(fi + 2, j)];
-        }
-        limiter<NEQ>(lim, qll, ql, qr, qrr);
-      }
-      err |= riemann<NEQ>(p, ql, qr, ff);
+      limiter<NEQ>(lim, qll, ql, qr, qrr);
+    }
+    err |= riemann<NEQ>(p, ql, qr, ff);
+    if (!yf) {
+      const int fi = f % 17, j = f / 17 + 1;
 #pragma unroll
       for (int k = 0; k < NEQ; ++k) F[k * 272 + (j - 1) * 17 + fi] = ff[k];
     } else {
       const int g = f - 272;
       const int i = g % 16 + 1, fj = g / 16;
 #pragma unroll
-      for (int k = 0; k < NEQ; ++k) {
-        ql[k] = prim[k * NC + IDX(i, fj)];
-        qr[k] = prim[k * NC + IDX(i, fj + 1)];
-      }
-      double t;
-      t = ql[1]; ql[1] = ql[2]; ql[2] = t;   // swapy, src/swapy.f90:4-13
-      t = qr[1]; qr[1] = qr[2]; qr[2] = t;
-      if (CHOICE == 2) {
-        double qll[NEQ], qrr[NEQ];
-#pragma unroll
-        for (int k = 0; k < NEQ; ++k) {
-          qll[k] = prim[k * NC + IDX(i, fj - 1)];
-          qrr[k] = prim[k * NC + IDX(i, fj + 2)];
-        }
-        t = qll[1]; qll[1] = qll[2]; qll[2] = t;
-        t = qrr[1]; qrr[1] = qrr[2]; qrr[2] = t;
-        limiter<NEQ>(lim, qll, ql, qr, qrr);
-      }
-      err |= riemann<NEQ>(p, ql, qr, ff);
-      t = ff[1]; ff[1] = ff[2]; ff[2] = t;
-#pragma unroll
-      for (int k = 0; k < NEQ; ++k) G[k * 272 + fj * 16 + (i - 1)] = ff[k];
+      for (int k = 0; k < NEQ; ++k) G[k * 272 + fj * 16 + (i - 1)] = ff[(k == 1 || k == 2) ? 3 - k : k];
     }
   }
   return err;
@@ -275,6 +269,22 @@ __global__ __launch_bounds__(BS) void k_stageB(const wx_params *__restrict__ P, 
   const double *qb = Q + (size_t)b * NEQ * NC;
   double *ub = U + (size_t)b * NEQ * NC;
   const int tid = threadIdx.x;
+  // two-layer ghost ring: 4 sides x 2 layers x 16 positions = 128 cells,
+  // gathered by waves of their own when the workgroup has them.  The op of
+  // the ghost's segment and Uⁿ of the thread's interior cell (used by the
+  // update after the sweep) are read first, so that their latency overlaps
+  // the interior's primitive recovery.
+  const int gt = BS >= 384 ? tid - 256 : tid;
+  const bool ghost = gt >= 0 && gt < 128;
+  const int side = gt >> 5, layer = ((gt >> 4) & 1) + 1, pos = (gt & 15) + 1;
+  int2 op = make_int2(0, 0);
+  if (ghost) op = ops[r * 8 + side * 2 + ((pos - 1) >> 3)];
+  double u0[NEQ];
+  if (tid < 256) {
+    const int i = tid % 16 + 1, j = tid / 16 + 1;
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) u0[k] = ub[k * NC + IDX(i, j)];
+  }
   // interior of U^{n+1/2} (imposed when the block is in the source list)
   if (tid < 256) {
     const int i = tid % 16 + 1, j = tid / 16 + 1;
@@ -286,15 +296,9 @@ __global__ __launch_bounds__(BS) void k_stageB(const wx_params *__restrict__ P, 
 #pragma unroll
     for (int k = 0; k < NEQ; ++k) prim[k * NC + IDX(i, j)] = q[k];
   }
-  // two-layer ghost ring: 4 sides x 2 layers x 16 positions = 128 cells,
-  // gathered by waves of their own when the workgroup has them
   double gh[NEQ];
   int gi = 0, gj = 0;
-  const int gt = BS >= 384 ? tid - 256 : tid;
-  const bool ghost = gt >= 0 && gt < 128;
   if (ghost) {
-    const int side = gt >> 5, layer = ((gt >> 4) & 1) + 1, pos = (gt & 15) + 1;
-    const int2 op = ops[r * 8 + side * 2 + ((pos - 1) >> 3)];
     ghost_gather<NEQ>(p, Q, binfo, op, b, L.level, L.icx, L.icy, side, layer, pos, vjet, gh);
     ghost_coords(side, layer, pos, gi, gj);
     double q[NEQ], T;
@@ -318,11 +322,11 @@ __global__ __launch_bounds__(BS) void k_stageB(const wx_params *__restrict__ P, 
 #pragma unroll
     for (int k = 0; k < NEQ; ++k) {
       const double *uk = ub + k * NC;
-      double v = uk[IDX(i, j)] - dtdx * (F[k * 272 + (j - 1) * 17 + i] - F[k * 272 + (j - 1) * 17 + i - 1])
+      double v = u0[k] - dtdx * (F[k * 272 + (j - 1) * 17 + i] - F[k * 272 + (j - 1) * 17 + i - 1])
                  - dtdy * (G[k * 272 + j * 16 + (i - 1)] - G[k * 272 + (j - 1) * 16 + (i - 1)]);
       if (p->cylindrical) v = v + dt * Sc[k];
       if (visc)   // src/viscosity.f90:6-25 (no dt factor)
-        v = v + eta * (uk[IDX(i + 1, j)] + uk[IDX(i - 1, j)] + uk[IDX(i, j + 1)] + uk[IDX(i, j - 1)] - 4. * uk[IDX(i, j)]);
+        v = v + eta * (uk[IDX(i + 1, j)] + uk[IDX(i - 1, j)] + uk[IDX(i, j + 1)] + uk[IDX(i, j - 1)] - 4. * u0[k]);
       un[k] = v;
     }
   }
