@@ -24,6 +24,8 @@ run's (the plain path's with --path plain); `plain`, `batched` and
   --batch B         jet variants in the ensemble (default 16)
   --path plain|batched   one path only (default: both)
   --no-surrogate    skip the batched+surrogate run
+  --groups G        concurrently stepped groups of the ensemble (default 4)
+  --no-tune --eager default launch shapes / no hipGraphs (for counter runs)
   --no-cpu-baseline skip the CPU oracle timing
 """
 from __future__ import annotations
@@ -139,12 +141,13 @@ def _final_state(sim):
     return u, lp, leaf, ic
 
 
-def run_batched(cfgs, steps, warmup, model, nstates=2):
-    """The batched path: every instance in one ensemble, surrogate cooling
-    (when a model is given), autotuned launch shapes, steps replayed from
-    hipGraphs.  Returns the final states of the first `nstates` instances."""
+def run_batched(cfgs, steps, warmup, model, nstates=2, groups=4, tune=True, graph=True):
+    """The batched path: every instance in one ensemble of `groups`
+    concurrently stepped groups, surrogate cooling (when a model is given),
+    autotuned launch shapes, steps replayed from hipGraphs.  Returns the
+    final states of the first `nstates` instances."""
     from walicxe2d_amd.capi import Ensemble
-    ens = Ensemble(cfgs)
+    ens = Ensemble(cfgs, groups=groups)
     ens.basegrid()
     for _ in range(4):
         ens.admesh()
@@ -152,8 +155,10 @@ def run_batched(cfgs, steps, warmup, model, nstates=2):
         ens.set_surrogate(model)
     if warmup:
         ens.step(warmup)
-    grids, threads, _ = ens.tune(reps=3)
-    ens.set_graph(True)
+    grids, threads = [0, 0, 0], [256, 256, 256]
+    if tune:
+        grids, threads, _ = ens.tune(reps=3)
+    ens.set_graph(graph)
     ens.step(0)
     ens.sur_counters(reset=True)
     t0 = time.perf_counter()
@@ -185,6 +190,9 @@ def main():
     ap.add_argument("--batch", type=int, default=16, help="jet variants in the batched ensemble")
     ap.add_argument("--path", choices=["both", "plain", "batched"], default="both")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the batched+surrogate run")
+    ap.add_argument("--groups", type=int, default=4, help="concurrently stepped groups of the ensemble")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shapes (one workgroup per block)")
+    ap.add_argument("--eager", action="store_true", help="no hipGraphs: every launch from the host")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -229,7 +237,8 @@ def main():
         cfgs = [P.config_c5(100 + k) for k in range(args.batch)]
         # the batched path with the exact cooling: bit-identical to single-mesh
         # runs of the same variants (so its L1 error is zero by construction)
-        bx = run_batched(cfgs, args.steps, args.warmup, None, nstates=args.batch)
+        bopts = dict(groups=args.groups, tune=not args.no_tune, graph=not args.eager)
+        bx = run_batched(cfgs, args.steps, args.warmup, None, nstates=args.batch, **bopts)
         batched = {"value": bx["value"], "ms_per_step": bx["ms_per_step"],
                    "leaf_blocks_end_total": bx["nleaf"], "persistent_grids": bx["grids"],
                    "threads_per_workgroup": bx["threads"],
@@ -238,7 +247,7 @@ def main():
         model = S.SurrogateModel.load() if os.path.exists(S.MODEL_PATH) else None
         sur = None
         if model is not None and not args.no_surrogate:
-            bs = run_batched(cfgs, args.steps, args.warmup, model, nstates=args.batch)
+            bs = run_batched(cfgs, args.steps, args.warmup, model, nstates=args.batch, **bopts)
             # L1 density error of every instance against its exact run
             l1 = [l1_density(cfgs[i], bs["states"][i], bx["states"][i]) for i in range(len(cfgs))]
             sur = {"value": bs["value"], "ms_per_step": bs["ms_per_step"],
@@ -256,7 +265,8 @@ def main():
         out["config"].update({"path": best_path, "instances": args.batch,
                               "variants": "jet seeds 777+100..777+%d" % (99 + args.batch),
                               "leaf_blocks_end_total": best["nleaf"], "persistent_grids": best["grids"],
-                              "threads_per_workgroup": best["threads"], "graph": True})
+                              "threads_per_workgroup": best["threads"], "graph": not args.eager,
+                              "groups": args.groups})
         out["roofline"] = best_prof
         out["batched"] = batched
         out["surrogate"] = sur

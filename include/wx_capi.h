@@ -148,8 +148,9 @@ int wx_set_time(wx_ctx *ctx, double time);
  * every launch (no counterpart in the reference, which has no timers).
  * Returns the accumulated milliseconds / launch counts of kernels
  * 0 stage A, 1 stage B, 2 boundaryI+cooling, 3 dt reduction, 4 block
- * statistics, 5 refine/coarse ops; 6 host wall time waiting for the flags and
- * running admesh, 7 host wall time compiling and uploading the ghost lists
+ * statistics, 5 refine/coarse ops; 6 host wall time running admesh on the
+ * flags (after the wait for the step), 7 host wall time compiling and
+ * uploading the ghost lists
  * (n entries); then, if enable >= 0, resets
  * the counters and switches recording on (1) or off (0). */
 int wx_profile(wx_ctx *ctx, int32_t enable, double *ms, int64_t *counts, int32_t n);
@@ -170,6 +171,15 @@ int wx_info(wx_ctx *ctx, int32_t *nbmax, int32_t *nbleafs, int32_t *ninternal);
  */
 typedef struct wx_ens wx_ens;
 int wx_ens_create(const wx_params *p, int32_t n, wx_ens **out);
+/* The same with the instances split into `groups` groups of consecutive
+ * instances (1 <= groups <= n), each with its own stream, queue, graphs and
+ * worker threads (WX_HOST_THREADS / groups each).  wx_ens_step then steps
+ * the groups concurrently from one host thread each: one group's admesh and
+ * list compilation overlap the other groups' kernels, and the groups'
+ * kernels overlap one another.  Results are bit-identical for any grouping.
+ * wx_ens_create is the one-group case. */
+int wx_ens_create_groups(const wx_params *p, int32_t n, int32_t groups, wx_ens **out);
+int wx_ens_groups(wx_ens *ens, int32_t *groups);
 int wx_ens_destroy(wx_ens *ens);
 /* Root layout + initial data of instance inst (as wx_basegrid). */
 int wx_ens_basegrid(wx_ens *ens, int32_t inst, const double *uroots, int32_t nroots);
@@ -192,10 +202,15 @@ int wx_ens_set_graph(wx_ens *ens, int32_t enable);
 int wx_ens_set_launch(wx_ens *ens, int32_t kernel, int32_t grid, int32_t threads);
 /* Autotuner: times the three stage kernels at every workgroup size and
  * several persistent grids on the current meshes (reps launches each) and
- * keeps the fastest; returns the chosen grids, threads and microseconds per
- * launch (3 each).  Later steps give the same results as untuned ones. */
+ * keeps the fastest, for each group; returns group 0's chosen grids, threads
+ * and microseconds per launch (3 each).  Later steps give the same results
+ * as untuned ones. */
 int wx_ens_tune(wx_ens *ens, int32_t reps, int32_t *grids, int32_t *threads, double *us);
+/* As wx_profile, summed over the groups.  While recording is on, wx_ens_step
+ * steps the groups one after the other, so that each kernel's time is not
+ * shared with another group's kernels. */
 int wx_ens_profile(wx_ens *ens, int32_t enable, double *ms, int64_t *counts, int32_t n);
+/* Instances, leaf blocks of all instances, group 0's launch shapes. */
 int wx_ens_info(wx_ens *ens, int32_t *ninst, int32_t *nleaf_total, int32_t *grids, int32_t *threads);
 int wx_last_error(char *buf, int32_t len);
 /* HIP stream the context launches on (hipStream_t as void*). */

@@ -5,14 +5,25 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 #include "../../walicxe2d_amd/csrc/wx_surrogate.h"
 
+// Flags by slot; with an ensemble split in groups, one table per group,
+// the group of an instance read from its jet[11] (a field the jet-less
+// parameters leave unused).
 static std::vector<signed char> g_flags;
+static std::vector<std::vector<signed char>> g_gflags;
 static std::vector<wx::MeshOp> g_ops;
+static std::mutex g_mu;
 
 extern "C" void wxsim_set_flags(const signed char *f, int n) { g_flags.assign(f, f + n); }
+extern "C" void wxsim_set_group_flags(int group, const signed char *f, int n) {
+  if ((int)g_gflags.size() <= group) g_gflags.resize(group + 1);
+  g_gflags[group].assign(f, f + n);
+}
 extern "C" int wxsim_ops(int *out, int cap) {
+  std::lock_guard<std::mutex> g(g_mu);
   int n = (int)g_ops.size();
   for (int k = 0; k < n && k < cap; ++k) {
     out[6 * k] = g_ops[k].kind; out[6 * k + 1] = g_ops[k].a; out[6 * k + 2] = g_ops[k].s1;
@@ -23,10 +34,11 @@ extern "C" int wxsim_ops(int *out, int cap) {
 }
 
 namespace wx {
-static void write_flags(const LeafRec *leaves, int n, signed char *flag, double *stat) {
+static void write_flags(const wx_params *P, const LeafRec *leaves, int n, signed char *flag, double *stat) {
   for (int l = 0; l < n; ++l) {
     int id = leaves[l].id;
-    flag[id] = id < (int)g_flags.size() ? g_flags[id] : 0;
+    const std::vector<signed char> &tab = g_gflags.empty() ? g_flags : g_gflags[(int)P[leaves[l].inst].jet[11]];
+    flag[id] = id < (int)tab.size() ? tab[id] : 0;
     stat[2 * id] = 1.0;
     stat[2 * id + 1] = 1e30;
   }
@@ -61,16 +73,16 @@ int launch_stageB(const wx_params *, int, QueueRef q, int grid, int bs, int pers
   check_shape(q, grid, bs, persist);
   return 0;
 }
-int launch_fillI_cool(const wx_params *, int, QueueRef q, int grid, int bs, int persist, const int2 *,
+int launch_fillI_cool(const wx_params *P, int, QueueRef q, int grid, int bs, int persist, const int2 *,
                       const BlockRec *, const double *, double *, double *stat, int8_t *flag, DevScalars *,
                       hipStream_t) {
   check_shape(q, grid, bs, persist);
-  write_flags(q.rec, q.hdr->nleaf, (signed char *)flag, stat);
+  write_flags(P, q.rec, q.hdr->nleaf, (signed char *)flag, stat);
   return 0;
 }
-int launch_blockstat(const wx_params *, int, const LeafRec *leaves, int nleaf, const double *, double *stat,
+int launch_blockstat(const wx_params *P, int, const LeafRec *leaves, int nleaf, const double *, double *stat,
                      int8_t *flag, hipStream_t) {
-  write_flags(leaves, nleaf, (signed char *)flag, stat);
+  write_flags(P, leaves, nleaf, (signed char *)flag, stat);
   return 0;
 }
 int launch_dtreduce(const wx_params *, const LeafRec *, const double *, const int32_t *, int ninst,
@@ -88,6 +100,7 @@ int launch_gather_flags(QueueRef q, int, const int8_t *flag, int8_t *flagq, cons
   return 0;
 }
 int launch_meshops(int, const MeshOp *ops, int nops, double *, hipStream_t) {
+  std::lock_guard<std::mutex> g(g_mu);
   for (int k = 0; k < nops; ++k) g_ops.push_back(ops[k]);
   return 0;
 }

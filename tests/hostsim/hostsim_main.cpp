@@ -17,13 +17,14 @@
 
 extern "C" void wxsim_set_flags(const signed char *f, int n);
 extern "C" int wxsim_ops(int *out, int cap);
+extern "C" void wxsim_set_group_flags(int group, const signed char *f, int n);
 
-static int run_ensemble(int rounds, unsigned seed);
+static int run_ensemble(int rounds, unsigned seed, int groups);
 
 int main(int argc, char **argv) {
   const int rounds = argc > 1 ? atoi(argv[1]) : 200;
   const unsigned seed = argc > 2 ? (unsigned)atoi(argv[2]) : 1u;
-  if (argc > 3 && std::string(argv[3]) == "ens") return run_ensemble(rounds, seed);
+  if (argc > 3 && std::string(argv[3]) == "ens") return run_ensemble(rounds, seed, argc > 4 ? atoi(argv[4]) : 1);
   const int cap = 30000, nrx = 8, nry = 8, nlevs = 5, neq = 4;
   wx_params p{};
   p.neq = neq; p.nlevs = nlevs; p.nbrootx = nrx; p.nbrooty = nry; p.nblocks = cap;
@@ -95,8 +96,11 @@ int main(int argc, char **argv) {
 }
 
 // ---------------------------------------------------------------------
-// Ensemble mode: instance i's block ID nb sits at slot base_i + nb with
-// base_i = sum over j < i of (nblocks_j + 1) (the ensemble's slot layout).
+// Ensemble mode: the instances are split into `groups` groups of
+// consecutive instances (n / groups each, the first n % groups one more);
+// within its group, instance i's block ID nb sits at slot base_i + nb with
+// base_i = sum over the group's earlier instances j of (nblocks_j + 1).
+// Each group reads its flags from a table of its own (jet[11] = group).
 struct OMesh {
   wx_params p{};
   std::vector<double> u, up, pr;
@@ -113,12 +117,15 @@ struct OMesh {
   }
 };
 
-static int run_ensemble(int rounds, unsigned seed) {
+static int run_ensemble(int rounds, unsigned seed, int groups) {
   const int n = 3, neq = 4;
   const int nrx[n] = {8, 4, 2}, nry[n] = {8, 2, 6}, nl[n] = {5, 3, 4}, caps[n] = {30000, 4000, 10000};
   std::vector<wx_params> P(n);
-  std::vector<int> base(n);
-  int nslots = 0;
+  std::vector<int> base(n), grp(n);
+  if (groups < 1 || groups > n) { fprintf(stderr, "groups %d\n", groups); return 2; }
+  for (int k = 0, i = 0; k < groups; ++k)
+    for (int c = 0; c < n / groups + (k < n % groups ? 1 : 0); ++c) grp[i++] = k;
+  std::vector<int> gslots(groups, 0);
   for (int i = 0; i < n; ++i) {
     wx_params &p = P[i];
     p = wx_params{};
@@ -126,11 +133,12 @@ static int run_ensemble(int rounds, unsigned seed) {
     p.cv = 1.5; p.gamma = 5. / 3.; p.cfl = 0.4; p.eta = 1e-3; p.tol = 0.05; p.tol2 = 1.0;
     p.compat = 3; p.Tempsc = 1; p.Psc = 1; p.tsc = 1;
     for (int l = 0; l < nl[i]; ++l) { p.dx[l] = 1.0 / (16 * nrx[i]) / (1 << l); p.dy[l] = 1.0 / (16 * nry[i]) / (1 << l); }
-    base[i] = nslots;
-    nslots += caps[i] + 1;
+    p.jet[11] = grp[i];
+    base[i] = gslots[grp[i]];
+    gslots[grp[i]] += caps[i] + 1;
   }
   wx_ens *e = nullptr;
-  if (wx_ens_create(P.data(), n, &e)) { fprintf(stderr, "ens create failed\n"); return 2; }
+  if (wx_ens_create_groups(P.data(), n, groups, &e)) { fprintf(stderr, "ens create failed\n"); return 2; }
   std::vector<OMesh> om(n);
   for (int i = 0; i < n; ++i) {
     std::vector<double> u((size_t)(nrx[i] * nry[i] + 1) * neq * 400, 1.0);
@@ -141,7 +149,8 @@ static int run_ensemble(int rounds, unsigned seed) {
   std::uniform_real_distribution<double> U01(0.0, 1.0);
   int total_ops = 0;
   for (int r = 0; r < rounds; ++r) {
-    std::vector<signed char> f(nslots, 0);
+    std::vector<std::vector<signed char>> f(groups);
+    for (int k = 0; k < groups; ++k) f[k].assign(gslots[k], 0);
     std::vector<std::vector<int32_t>> up(n), dn(n);
     for (int i = 0; i < n; ++i) {
       wxo_mesh &m = om[i].m;
@@ -152,11 +161,11 @@ static int run_ensemble(int rounds, unsigned seed) {
         if (!om[i].leaf[nb]) continue;
         const int lev = om[i].lp[(size_t)nb * 8];
         const double x = U01(rng);
-        if (x < pup && lev < nl[i]) { f[base[i] + nb] = 1; up[i][nb] = 1; }
-        else if (x < pup + pdn) { f[base[i] + nb] = 2; dn[i][nb] = 1; }
+        if (x < pup && lev < nl[i]) { f[grp[i]][base[i] + nb] = 1; up[i][nb] = 1; }
+        else if (x < pup + pdn) { f[grp[i]][base[i] + nb] = 2; dn[i][nb] = 1; }
       }
     }
-    wxsim_set_flags(f.data(), nslots);
+    for (int k = 0; k < groups; ++k) wxsim_set_group_flags(k, f[k].data(), gslots[k]);
     double z = 0;
     if (wx_ens_step(e, 1, 1, &z)) {
       char b[512];
@@ -192,7 +201,8 @@ static int run_ensemble(int rounds, unsigned seed) {
       }
     }
   }
-  printf("hostsim ensemble: %d instances, %d rounds, %d mesh ops -- trees identical\n", n, rounds, total_ops);
+  printf("hostsim ensemble: %d instances in %d groups, %d rounds, %d mesh ops -- trees identical\n", n, groups, rounds,
+         total_ops);
   wx_ens_destroy(e);
   return 0;
 }

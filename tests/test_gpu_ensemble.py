@@ -42,8 +42,8 @@ def _compare_instance(ens, i, sim, label):
     return dt_e
 
 
-def _run_pair(cfgs, nsteps, label):
-    ens = Ensemble(cfgs)
+def _run_pair(cfgs, nsteps, label, groups=1, graph=False):
+    ens = Ensemble(cfgs, groups=groups)
     ens.basegrid()
     sims = []
     for c in cfgs:
@@ -56,6 +56,7 @@ def _run_pair(cfgs, nsteps, label):
             ens.admesh()
             for s in sims:
                 s.admesh()
+    ens.set_graph(graph)
     for step in range(nsteps):
         ens.step(1, admesh=amr)
         for s, c in zip(sims, cfgs):
@@ -84,8 +85,8 @@ def test_jet_variant_ensemble_matches_single_runs():
     _run_pair(cfgs, 8, "c5")
 
 
-def _ens(cfgs, graph=False):
-    e = Ensemble(cfgs)
+def _ens(cfgs, graph=False, groups=1):
+    e = Ensemble(cfgs, groups=groups)
     e.basegrid()
     for _ in range(3):
         e.admesh()
@@ -166,5 +167,29 @@ def test_jet_launch_shapes_keep_results():
     a.step(6)
     b.step(6)
     _same_state(a, b, len(cfgs), "jet launch shapes")
+    a.close()
+    b.close()
+
+
+def test_grouped_explosion_ensemble_matches_single_runs():
+    """The five explosion instances split into three groups (2, 2, 1) stepped
+    concurrently from host threads, each group on its own stream, with graph
+    replays: each instance equals its own single-mesh run bit for bit."""
+    cfgs = [_c4(i) for i in C4_PICK]
+    n, nleaf = _run_pair(cfgs, 10, "c4 grouped", groups=3, graph=True)
+    assert n == len(cfgs) and nleaf > 0
+
+
+def test_grouped_jet_ensemble_matches_ungrouped():
+    """Four jet variants in two concurrent groups against the same four in one
+    group, over 12 steps with admesh: identical states."""
+    cfgs = [P.config_c5(k, nbrootx=16, nbrooty=4, nblocks=4000) for k in range(4)]
+    a = _ens(cfgs, graph=True)
+    b = _ens(cfgs, graph=True, groups=2)
+    for step in range(3):
+        a.step(4)
+        b.step(4)
+        _same_state(a, b, len(cfgs), f"groups step {step}")
+    assert a.info()[1] == b.info()[1]
     a.close()
     b.close()

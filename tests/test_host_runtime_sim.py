@@ -22,12 +22,27 @@ def test_host_tree_matches_oracle_under_asan(seed):
 
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not installed")
-def test_ensemble_trees_match_oracle_under_asan():
+@pytest.mark.parametrize("groups", [1, 3])
+def test_ensemble_trees_match_oracle_under_asan(groups):
     """Three meshes of different root layouts, depths and capacities advanced as
-    one ensemble (worker threads on): every instance's tree after every admesh
+    one ensemble (worker threads on), in one group or in three groups stepped
+    from concurrent host threads: every instance's tree after every admesh
     equals the oracle's for the same flags."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
-    r = subprocess.run([os.path.join(HERE, "hostsim_main"), "120", "5", "ens"], capture_output=True, text=True,
-                       timeout=600, env={**os.environ, "WX_HOST_THREADS": "4"})
+    r = subprocess.run([os.path.join(HERE, "hostsim_main"), "120", "5", "ens", str(groups)], capture_output=True,
+                       text=True, timeout=600, env={**os.environ, "WX_HOST_THREADS": "4"})
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    assert "trees identical" in r.stdout
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not installed")
+@pytest.mark.parametrize("groups", [2, 3])
+def test_grouped_ensemble_under_tsan(groups):
+    """The grouped ensemble under ThreadSanitizer: the groups' host threads
+    and their worker pools share no unsynchronised state."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    r = subprocess.run([os.path.join(HERE, "hostsim_tsan"), "40", "3", "ens", str(groups)], capture_output=True,
+                       text=True, timeout=600, env={**os.environ, "WX_HOST_THREADS": "6"})
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    assert "ThreadSanitizer" not in r.stderr, r.stderr[-4000:]
     assert "trees identical" in r.stdout

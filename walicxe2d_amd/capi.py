@@ -30,7 +30,7 @@ EXPORTS = [
     "wx_last_error", "wx_stream", "wx_profile",
     "wx_ens_create", "wx_ens_destroy", "wx_ens_basegrid", "wx_ens_admesh", "wx_ens_step", "wx_ens_instance",
     "wx_ens_download", "wx_ens_get_tree", "wx_ens_set_graph", "wx_ens_set_launch", "wx_ens_tune",
-    "wx_ens_profile", "wx_ens_info",
+    "wx_ens_profile", "wx_ens_info", "wx_ens_create_groups", "wx_ens_groups",
     "wx_sur_pairs", "wx_sur_features", "wx_sur_infer", "wx_sur_grad", "wx_sur_train", "wx_set_surrogate",
     "wx_ens_set_surrogate", "wx_sur_counters", "wx_ens_sur_counters",
 ]
@@ -85,6 +85,8 @@ def load_library(path: str = LIBPATH) -> ctypes.CDLL:
     L.wx_stream.restype = P
     L.wx_profile.argtypes = [P, ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]
     L.wx_ens_create.argtypes = [ctypes.POINTER(WxParams), ctypes.c_int32, ctypes.POINTER(P)]
+    L.wx_ens_create_groups.argtypes = [ctypes.POINTER(WxParams), ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(P)]
+    L.wx_ens_groups.argtypes = [P, _ip]
     L.wx_ens_destroy.argtypes = [P]
     L.wx_ens_basegrid.argtypes = [P, ctypes.c_int32, _dp, ctypes.c_int32]
     L.wx_ens_admesh.argtypes = [P, _ip]
@@ -298,9 +300,10 @@ class Simulation:
 class Ensemble:
     """Many independent AMR meshes (one Config each, same neq) advanced
     together on the GPU behind the wx_ens_* entry points: each stage is one
-    persistent launch over all instances' leaf blocks."""
+    launch over all instances' leaf blocks of a group; `groups` groups of
+    consecutive instances are stepped concurrently, each on its own stream."""
 
-    def __init__(self, cfgs):
+    def __init__(self, cfgs, groups: int = 1):
         self.L = load_library()
         self.cfgs = list(cfgs)
         self.n = len(self.cfgs)
@@ -309,8 +312,9 @@ class Ensemble:
         self.neq = self.cfgs[0].neq
         arr = (WxParams * self.n)(*[c.params() for c in self.cfgs])
         h = ctypes.c_void_p()
-        _check(self.L.wx_ens_create(arr, self.n, ctypes.byref(h)), "wx_ens_create")
+        _check(self.L.wx_ens_create_groups(arr, self.n, groups, ctypes.byref(h)), "wx_ens_create_groups")
         self.h = h
+        self.groups = groups
 
     def close(self):
         if getattr(self, "h", None):

@@ -23,6 +23,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "wx_runtime.h"
@@ -30,8 +31,9 @@
 
 using namespace wx;
 
-struct wx_ens : Runtime {
+struct EnsGroup : Runtime {
   int n = 0, neq = 0;
+  int ibase = 0;   // ensemble index of the group's first instance
   std::vector<Mesh> m;
   std::vector<int32_t> slotbase, leafbase;   // per instance
   size_t nslots = 0, nrec = 0;
@@ -86,13 +88,13 @@ struct wx_ens : Runtime {
 
 namespace {
 
-int collect_errors(wx_ens *e, const std::vector<int> &which) {
+int collect_errors(EnsGroup *e, const std::vector<int> &which) {
   for (int i : which)
-    if (e->rcs[i]) return fail(e->rcs[i], "instance %d: %s", i, e->errs[i].c_str());
+    if (e->rcs[i]) return fail(e->rcs[i], "instance %d: %s", e->ibase + i, e->errs[i].c_str());
   return WX_OK;
 }
 
-void drop_graphs(wx_ens *e) {
+void drop_graphs(EnsGroup *e) {
   for (int k = 0; k < 2; ++k) {
     if (e->gexec[k]) (void)hipGraphExecDestroy(e->gexec[k]);
     if (e->graph[k]) (void)hipGraphDestroy(e->graph[k]);
@@ -106,21 +108,21 @@ void drop_graphs(wx_ens *e) {
 struct Shape {
   int grid, persist;
 };
-Shape stage_grid(const wx_ens *e, int k, bool for_graph) {
+Shape stage_grid(const EnsGroup *e, int k, bool for_graph) {
   if (e->grid[k] > 0) return {e->grid[k], 1};
   return {for_graph ? e->gspan : e->nleaf_total, 0};
 }
 
 // Queue span a graph is captured for: the queue length and an eighth more,
 // so that small mesh changes replay the same graph.
-int graph_span(const wx_ens *e) {
+int graph_span(const EnsGroup *e) {
   const size_t want = ((size_t)e->nleaf_total * 9 / 8 + 255) / 256 * 256;
   return (int)std::min(e->nrec, std::max<size_t>(want, 256));
 }
 
 // Recompiles the tables of instances whose mesh changed and uploads their
 // regions, the leaf-count prefix sums, and rebuilds the queue index.
-int ensure_queue(wx_ens *e) {
+int ensure_queue(EnsGroup *e) {
   std::vector<int> stale;
   for (int i = 0; i < e->n; ++i)
     if (!e->m[i].tables_valid) stale.push_back(i);
@@ -191,7 +193,7 @@ int ensure_queue(wx_ens *e) {
 
 // Block statistics of new leaves (or of every leaf of an instance whose
 // statistics are all stale), one launch for all instances.
-int ensure_stats(wx_ens *e) {
+int ensure_stats(EnsGroup *e) {
   int rc = ensure_queue(e);
   if (rc) return rc;
   e->dirty.clear();
@@ -227,7 +229,7 @@ int ensure_stats(wx_ens *e) {
 
 // dt reduction (time += dt), the three stages and the flag gather, reading
 // U[par] and writing U[par ^ 1]; `prof` records kernel events.
-int launch_step(wx_ens *e, int par, bool for_graph) {
+int launch_step(EnsGroup *e, int par, bool for_graph) {
   const bool prof = !for_graph;
   double *Ucur = e->U[par].ptr, *Uoth = e->U[par ^ 1].ptr;
   const QueueRef q = e->queue();
@@ -264,7 +266,7 @@ int launch_step(wx_ens *e, int par, bool for_graph) {
   return WX_OK;
 }
 
-int ensure_graphs(wx_ens *e) {
+int ensure_graphs(EnsGroup *e) {
   if (e->gexec[0] && e->gexec[1] && e->nleaf_total <= e->gspan && 2 * e->nleaf_total > e->gspan) return WX_OK;
   drop_graphs(e);
   e->gspan = graph_span(e);
@@ -281,7 +283,7 @@ int ensure_graphs(wx_ens *e) {
   return WX_OK;
 }
 
-int check_scalars(wx_ens *e) {
+int check_scalars(EnsGroup *e) {
   for (int i = 0; i < e->n; ++i)
     if (e->hS[i].err)
       return fail(WX_EHLLC, "Error in hllc (instance %d, device flag %d): a Riemann problem had no valid star speed", i,
@@ -290,9 +292,9 @@ int check_scalars(wx_ens *e) {
 }
 
 // admesh of every instance from the flags the last gather wrote (queue order).
-int admesh_all(wx_ens *e, int *changed) {
-  HostTimer ht(e, HID_ADMESH);
+int admesh_all(EnsGroup *e, int *changed) {
   HIPCHK(hipStreamSynchronize(e->stream));
+  HostTimer ht(e, HID_ADMESH);   // host work only (after the step's wait)
   int rc = check_scalars(e);
   if (rc) return rc;
   std::vector<int> all(e->n);
@@ -325,7 +327,7 @@ int admesh_all(wx_ens *e, int *changed) {
   return WX_OK;
 }
 
-int check_inst(wx_ens *e, int inst) {
+int check_inst(EnsGroup *e, int inst) {
   if (!e) return fail(WX_EARG, "null ensemble");
   if (inst < 0 || inst >= e->n) return fail(WX_EARG, "instance %d outside [0, %d)", inst, e->n);
   return WX_OK;
@@ -333,18 +335,12 @@ int check_inst(wx_ens *e, int inst) {
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int wx_ens_create(const wx_params *p, int32_t n, wx_ens **out) {
-  if (!p || !out || n < 1) return fail(WX_EARG, "bad argument");
-  std::string why;
-  for (int i = 0; i < n; ++i) {
-    if (!params_ok(p + i, why)) return fail(WX_EARG, "instance %d: bad parameters: %s", i, why.c_str());
-    if (p[i].neq != p[0].neq) return fail(WX_EARG, "instance %d: neq %d differs from instance 0 (%d)", i, p[i].neq, p[0].neq);
-  }
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(WX_EHIP, "no HIP device");
-  wx_ens *e = new wx_ens();
+int grp_destroy(EnsGroup *e);
+
+int grp_create(const wx_params *p, int32_t n, int nthreads, EnsGroup **out) {
+  EnsGroup *e = new EnsGroup();
   e->n = n;
   e->neq = p[0].neq;
   e->m.resize(n);
@@ -365,11 +361,6 @@ int wx_ens_create(const wx_params *p, int32_t n, wx_ens **out) {
   int dev = 0;
   (void)hipGetDevice(&dev);
   if (hipDeviceGetAttribute(&e->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || e->ncu < 1) e->ncu = 256;
-  int nthreads = 8;
-  if (const char *s = getenv("WX_HOST_THREADS")) nthreads = std::max(1, atoi(s));
-  else if (const char *s2 = getenv("OMP_NUM_THREADS")) nthreads = std::max(1, atoi(s2));
-  nthreads = std::min(nthreads, std::max(1, (int)std::thread::hardware_concurrency()));
-  nthreads = std::min(nthreads, 32);
   e->pool.reset(new WorkerPool(nthreads));
   int rc = e->create_stream();
   const size_t tiles = e->nslots * e->tile();
@@ -388,15 +379,15 @@ int wx_ens_create(const wx_params *p, int32_t n, wx_ens **out) {
   if (!rc) rc = e->dOpsI.ensure(e->nrec * 8);
   if (!rc) rc = e->dStat.ensure(e->nslots * 2);
   if (!rc) rc = e->dFlag.ensure(e->nslots);
-  if (rc) { wx_ens_destroy(e); return rc; }
+  if (rc) { grp_destroy(e); return rc; }
   if (hipHostMalloc((void **)&e->hS, (size_t)n * sizeof(DevScalars), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
       hipHostMalloc((void **)&e->hFlagQ, e->nrec, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
-    wx_ens_destroy(e);
+    grp_destroy(e);
     return fail(WX_ENOMEM, "pinned mirrors");
   }
   if (hipHostGetDevicePointer((void **)&e->hS_dev, e->hS, 0) != hipSuccess ||
       hipHostGetDevicePointer((void **)&e->hFlagQ_dev, e->hFlagQ, 0) != hipSuccess) {
-    wx_ens_destroy(e);
+    grp_destroy(e);
     return fail(WX_EHIP, "mapped pointers of the pinned mirrors");
   }
   memset(e->hS, 0, (size_t)n * sizeof(DevScalars));
@@ -408,12 +399,12 @@ int wx_ens_create(const wx_params *p, int32_t n, wx_ens **out) {
   (void)hipMemsetAsync(e->dBinfo.ptr, 0, e->nslots * sizeof(BlockRec), e->stream);
   (void)hipMemcpyAsync(e->dP.ptr, p, (size_t)n * sizeof(wx_params), hipMemcpyHostToDevice, e->stream);
   (void)hipMemcpyAsync(e->dLeafBase.ptr, e->leafbase.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, e->stream);
-  if (hipStreamSynchronize(e->stream) != hipSuccess) { wx_ens_destroy(e); return fail(WX_EHIP, "init sync"); }
+  if (hipStreamSynchronize(e->stream) != hipSuccess) { grp_destroy(e); return fail(WX_EHIP, "init sync"); }
   *out = e;
   return WX_OK;
 }
 
-int wx_ens_destroy(wx_ens *e) {
+int grp_destroy(EnsGroup *e) {
   if (!e) return WX_OK;
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   drop_graphs(e);
@@ -429,12 +420,12 @@ int wx_ens_destroy(wx_ens *e) {
   return WX_OK;
 }
 
-int wx_ens_basegrid(wx_ens *e, int32_t inst, const double *uroots, int32_t nroots) {
+int grp_basegrid(EnsGroup *e, int32_t inst, const double *uroots, int32_t nroots) {
   int rc = check_inst(e, inst);
   if (rc) return rc;
   Mesh &m = e->m[inst];
   if (!uroots || m.p.nbrootx * m.p.nbrooty != nroots)
-    return fail(WX_EARG, "instance %d: nroots %d != %d x %d", inst, nroots, m.p.nbrootx, m.p.nbrooty);
+    return fail(WX_EARG, "instance %d: nroots %d != %d x %d", e->ibase + inst, nroots, m.p.nbrootx, m.p.nbrooty);
   if ((rc = basegrid_tree(&m))) return rc;
   const size_t bytes = (size_t)(nroots + 1) * e->tile() * sizeof(double);
   HIPCHK(hipMemcpyAsync(e->cur_u() + (size_t)e->slotbase[inst] * e->tile(), uroots, bytes, hipMemcpyHostToDevice,
@@ -443,7 +434,7 @@ int wx_ens_basegrid(wx_ens *e, int32_t inst, const double *uroots, int32_t nroot
   return WX_OK;
 }
 
-int wx_ens_admesh(wx_ens *e, int32_t *changed) {
+int grp_admesh(EnsGroup *e, int32_t *changed) {
   if (!e) return fail(WX_EARG, "null ensemble");
   int rc = ensure_stats(e);
   if (rc) return rc;
@@ -456,10 +447,11 @@ int wx_ens_admesh(wx_ens *e, int32_t *changed) {
   return rc;
 }
 
-int wx_ens_step(wx_ens *e, int32_t nsteps, int32_t do_admesh, double *zone_updates) {
+int grp_step(EnsGroup *e, int32_t nsteps, int32_t do_admesh, double *zone_updates) {
   if (!e || nsteps < 0) return fail(WX_EARG, "bad argument");
   for (int i = 0; i < e->n; ++i)
-    if (e->m[i].nbleafs <= 0) return fail(WX_EARG, "instance %d has no mesh (call wx_ens_basegrid first)", i);
+    if (e->m[i].nbleafs <= 0)
+      return fail(WX_EARG, "instance %d has no mesh (call wx_ens_basegrid first)", e->ibase + i);
   double z = 0.0;
   int rc;
   for (int s = 0; s < nsteps; ++s) {
@@ -482,7 +474,7 @@ int wx_ens_step(wx_ens *e, int32_t nsteps, int32_t do_admesh, double *zone_updat
   return WX_OK;
 }
 
-int wx_ens_instance(wx_ens *e, int32_t inst, double *time, double *dt, int32_t *nbmax, int32_t *nbleafs) {
+int grp_instance(EnsGroup *e, int32_t inst, double *time, double *dt, int32_t *nbmax, int32_t *nbleafs) {
   int rc = check_inst(e, inst);
   if (rc) return rc;
   DevScalars s{};
@@ -495,7 +487,7 @@ int wx_ens_instance(wx_ens *e, int32_t inst, double *time, double *dt, int32_t *
   return WX_OK;
 }
 
-int wx_ens_download(wx_ens *e, int32_t inst, double *u) {
+int grp_download(EnsGroup *e, int32_t inst, double *u) {
   int rc = check_inst(e, inst);
   if (rc) return rc;
   if (!u) return fail(WX_EARG, "null buffer");
@@ -506,21 +498,21 @@ int wx_ens_download(wx_ens *e, int32_t inst, double *u) {
   return WX_OK;
 }
 
-int wx_ens_get_tree(wx_ens *e, int32_t inst, int32_t *lp, int32_t *leaf, int32_t *icoord, int32_t ld,
+int grp_get_tree(EnsGroup *e, int32_t inst, int32_t *lp, int32_t *leaf, int32_t *icoord, int32_t ld,
                     int32_t *nbmax, int32_t *nbleafs) {
   int rc = check_inst(e, inst);
   if (rc) return rc;
   return get_tree(&e->m[inst], lp, leaf, icoord, ld, nbmax, nbleafs);
 }
 
-int wx_ens_set_graph(wx_ens *e, int32_t enable) {
+int grp_set_graph(EnsGroup *e, int32_t enable) {
   if (!e) return fail(WX_EARG, "null ensemble");
   e->use_graph = enable != 0;
   if (!e->use_graph) drop_graphs(e);
   return WX_OK;
 }
 
-int wx_ens_set_launch(wx_ens *e, int32_t kernel, int32_t grid, int32_t threads) {
+int grp_set_launch(EnsGroup *e, int32_t kernel, int32_t grid, int32_t threads) {
   if (!e) return fail(WX_EARG, "null ensemble");
   if (kernel < 0 || kernel > 2 || grid < 0) return fail(WX_EARG, "kernel %d / grid %d", kernel, grid);
   if (threads != 256 && threads != 384 && threads != 512) return fail(WX_EARG, "threads %d (256, 384 or 512)", threads);
@@ -537,7 +529,7 @@ int wx_ens_set_launch(wx_ens *e, int32_t kernel, int32_t grid, int32_t threads) 
 // Uⁿ; stage B updates in place, so it runs on a scratch copy of Uⁿ.  The
 // statistics the fill's epilogue overwrites are recomputed afterwards, so
 // tuning leaves the results of later steps unchanged.
-int wx_ens_tune(wx_ens *e, int32_t reps, int32_t *grids, int32_t *threads, double *us) {
+int grp_tune(EnsGroup *e, int32_t reps, int32_t *grids, int32_t *threads, double *us) {
   if (!e || reps < 1) return fail(WX_EARG, "bad argument");
   int rc = ensure_stats(e);
   if (rc) return rc;
@@ -605,7 +597,7 @@ int wx_ens_tune(wx_ens *e, int32_t reps, int32_t *grids, int32_t *threads, doubl
   return rc;
 }
 
-int wx_ens_set_surrogate(wx_ens *e, const wx_sur_model *m) {
+int grp_set_surrogate(EnsGroup *e, const wx_sur_model *m) {
   if (!e) return fail(WX_EARG, "null ensemble");
   drop_graphs(e);
   if (!m) {
@@ -624,7 +616,7 @@ int wx_ens_set_surrogate(wx_ens *e, const wx_sur_model *m) {
   return WX_OK;
 }
 
-int wx_ens_sur_counters(wx_ens *e, int32_t reset, int64_t *cells, int64_t *fallbacks) {
+int grp_sur_counters(EnsGroup *e, int32_t reset, int64_t *cells, int64_t *fallbacks) {
   if (!e) return fail(WX_EARG, "null ensemble");
   if (!e->dSur.ptr) {
     if (cells) *cells = 0;
@@ -634,17 +626,246 @@ int wx_ens_sur_counters(wx_ens *e, int32_t reset, int64_t *cells, int64_t *fallb
   return sur_counters(e->dSur.ptr, e->stream, reset, cells, fallbacks);
 }
 
-int wx_ens_profile(wx_ens *e, int32_t enable, double *ms, int64_t *counts, int32_t n) {
+int grp_profile(EnsGroup *e, int32_t enable, double *ms, int64_t *counts, int32_t n) {
   if (!e) return fail(WX_EARG, "null ensemble");
   return prof_report(e, enable, ms, counts, n);
 }
 
-int wx_ens_info(wx_ens *e, int32_t *ninst, int32_t *nleaf_total, int32_t *grids, int32_t *threads) {
+int grp_info(EnsGroup *e, int32_t *ninst, int32_t *nleaf_total, int32_t *grids, int32_t *threads) {
   if (!e) return fail(WX_EARG, "null ensemble");
   if (ninst) *ninst = e->n;
   if (nleaf_total) *nleaf_total = e->nleaf_total;
   if (grids) for (int k = 0; k < 3; ++k) grids[k] = e->grid[k];
   if (threads) for (int k = 0; k < 3; ++k) threads[k] = e->bs[k];
+  return WX_OK;
+}
+
+}  // namespace
+
+// =====================================================================
+// The ensemble: its instances split into groups of consecutive instances,
+// each group with its own stream, queue, graphs and worker threads.  With
+// more than one group, wx_ens_step drives every group from a host thread of
+// its own, so that one group's host work (admesh, list compilation) runs
+// while the others' kernels occupy the GPU, and the groups' kernels overlap.
+struct wx_ens {
+  int n = 0, dev = 0;
+  std::vector<EnsGroup *> g;
+  std::vector<int> gi, li;   // group of each instance and its index there
+};
+
+namespace {
+
+int check_ens(wx_ens *e, int inst = 0) {
+  if (!e) return fail(WX_EARG, "null ensemble");
+  if (inst < 0 || inst >= e->n) return fail(WX_EARG, "instance %d outside [0, %d)", inst, e->n);
+  return WX_OK;
+}
+
+// Runs f(group) for every group, concurrently when there are several (each
+// on a host thread with the ensemble's device current); returns the first
+// failure with its message.
+template <class F>
+int each_group(wx_ens *e, bool concurrent, F f) {
+  const int G = (int)e->g.size();
+  if (!concurrent || G == 1) {
+    for (int k = 0; k < G; ++k) {
+      int rc = f(k);
+      if (rc) return rc;
+    }
+    return WX_OK;
+  }
+  std::vector<int> rcs(G, 0);
+  std::vector<std::string> msg(G);
+  std::vector<std::thread> th;
+  for (int k = 1; k < G; ++k)
+    th.emplace_back([&, k] {
+      (void)hipSetDevice(e->dev);
+      rcs[k] = f(k);
+      if (rcs[k]) msg[k] = last_error();
+    });
+  rcs[0] = f(0);
+  if (rcs[0]) msg[0] = last_error();
+  for (auto &t : th) t.join();
+  for (int k = 0; k < G; ++k)
+    if (rcs[k]) return fail(rcs[k], "%s", msg[k].c_str());
+  return WX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wx_ens_create(const wx_params *p, int32_t n, wx_ens **out) { return wx_ens_create_groups(p, n, 1, out); }
+
+int wx_ens_create_groups(const wx_params *p, int32_t n, int32_t groups, wx_ens **out) {
+  if (!p || !out || n < 1) return fail(WX_EARG, "bad argument");
+  if (groups < 1 || groups > n) return fail(WX_EARG, "groups %d outside [1, %d]", groups, n);
+  std::string why;
+  for (int i = 0; i < n; ++i) {
+    if (!params_ok(p + i, why)) return fail(WX_EARG, "instance %d: bad parameters: %s", i, why.c_str());
+    if (p[i].neq != p[0].neq) return fail(WX_EARG, "instance %d: neq %d differs from instance 0 (%d)", i, p[i].neq, p[0].neq);
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(WX_EHIP, "no HIP device");
+  int nthreads = 8;
+  if (const char *s = getenv("WX_HOST_THREADS")) nthreads = std::max(1, atoi(s));
+  else if (const char *s2 = getenv("OMP_NUM_THREADS")) nthreads = std::max(1, atoi(s2));
+  nthreads = std::min(nthreads, std::max(1, (int)std::thread::hardware_concurrency()));
+  nthreads = std::min(nthreads, 32);
+  wx_ens *e = new wx_ens();
+  e->n = n;
+  (void)hipGetDevice(&e->dev);
+  e->gi.resize(n);
+  e->li.resize(n);
+  int first = 0;
+  for (int k = 0; k < groups; ++k) {
+    const int cnt = n / groups + (k < n % groups ? 1 : 0);
+    EnsGroup *grp = nullptr;
+    int rc = grp_create(p + first, cnt, std::max(1, nthreads / groups), &grp);
+    if (rc) { wx_ens_destroy(e); return rc; }
+    grp->ibase = first;
+    e->g.push_back(grp);
+    for (int i = 0; i < cnt; ++i) { e->gi[first + i] = k; e->li[first + i] = i; }
+    first += cnt;
+  }
+  *out = e;
+  return WX_OK;
+}
+
+int wx_ens_destroy(wx_ens *e) {
+  if (!e) return WX_OK;
+  for (EnsGroup *g : e->g) grp_destroy(g);
+  delete e;
+  return WX_OK;
+}
+
+int wx_ens_basegrid(wx_ens *e, int32_t inst, const double *uroots, int32_t nroots) {
+  int rc = check_ens(e, inst);
+  return rc ? rc : grp_basegrid(e->g[e->gi[inst]], e->li[inst], uroots, nroots);
+}
+
+int wx_ens_admesh(wx_ens *e, int32_t *changed) {
+  int rc = check_ens(e);
+  if (rc) return rc;
+  int total = 0;
+  rc = each_group(e, false, [&](int k) {
+    int ch = 0;
+    int r = grp_admesh(e->g[k], &ch);
+    total += ch;
+    return r;
+  });
+  if (changed) *changed = total;
+  return rc;
+}
+
+int wx_ens_step(wx_ens *e, int32_t nsteps, int32_t do_admesh, double *zone_updates) {
+  int rc = check_ens(e);
+  if (rc) return rc;
+  // concurrent groups, except while kernel events are recorded: then each
+  // kernel's time is its own, not shared with another group's kernels
+  std::vector<double> z(e->g.size(), 0.0);
+  rc = each_group(e, !e->g[0]->prof, [&](int k) { return grp_step(e->g[k], nsteps, do_admesh, &z[k]); });
+  if (rc) return rc;
+  double zt = 0.0;
+  for (double v : z) zt += v;
+  if (zone_updates) *zone_updates = zt;
+  return WX_OK;
+}
+
+int wx_ens_instance(wx_ens *e, int32_t inst, double *time, double *dt, int32_t *nbmax, int32_t *nbleafs) {
+  int rc = check_ens(e, inst);
+  return rc ? rc : grp_instance(e->g[e->gi[inst]], e->li[inst], time, dt, nbmax, nbleafs);
+}
+
+int wx_ens_download(wx_ens *e, int32_t inst, double *u) {
+  int rc = check_ens(e, inst);
+  return rc ? rc : grp_download(e->g[e->gi[inst]], e->li[inst], u);
+}
+
+int wx_ens_get_tree(wx_ens *e, int32_t inst, int32_t *lp, int32_t *leaf, int32_t *icoord, int32_t ld,
+                    int32_t *nbmax, int32_t *nbleafs) {
+  int rc = check_ens(e, inst);
+  return rc ? rc : grp_get_tree(e->g[e->gi[inst]], e->li[inst], lp, leaf, icoord, ld, nbmax, nbleafs);
+}
+
+int wx_ens_set_graph(wx_ens *e, int32_t enable) {
+  int rc = check_ens(e);
+  return rc ? rc : each_group(e, false, [&](int k) { return grp_set_graph(e->g[k], enable); });
+}
+
+int wx_ens_set_launch(wx_ens *e, int32_t kernel, int32_t grid, int32_t threads) {
+  int rc = check_ens(e);
+  return rc ? rc : each_group(e, false, [&](int k) { return grp_set_launch(e->g[k], kernel, grid, threads); });
+}
+
+// Every group is tuned on its own meshes (one after the other, so that the
+// timings see an otherwise idle GPU); the values returned are group 0's.
+int wx_ens_tune(wx_ens *e, int32_t reps, int32_t *grids, int32_t *threads, double *us) {
+  int rc = check_ens(e);
+  return rc ? rc : each_group(e, false, [&](int k) {
+    return k == 0 ? grp_tune(e->g[k], reps, grids, threads, us) : grp_tune(e->g[k], reps, nullptr, nullptr, nullptr);
+  });
+}
+
+int wx_ens_set_surrogate(wx_ens *e, const wx_sur_model *m) {
+  int rc = check_ens(e);
+  return rc ? rc : each_group(e, false, [&](int k) { return grp_set_surrogate(e->g[k], m); });
+}
+
+int wx_ens_sur_counters(wx_ens *e, int32_t reset, int64_t *cells, int64_t *fallbacks) {
+  int rc = check_ens(e);
+  if (rc) return rc;
+  int64_t c = 0, f = 0;
+  rc = each_group(e, false, [&](int k) {
+    int64_t ck = 0, fk = 0;
+    int r = grp_sur_counters(e->g[k], reset, &ck, &fk);
+    c += ck;
+    f += fk;
+    return r;
+  });
+  if (cells) *cells = c;
+  if (fallbacks) *fallbacks = f;
+  return rc;
+}
+
+// Accumulated over the groups (with concurrent groups, kernel times add up
+// to more than the wall time).
+int wx_ens_profile(wx_ens *e, int32_t enable, double *ms, int64_t *counts, int32_t n) {
+  int rc = check_ens(e);
+  if (rc) return rc;
+  if (n < 0) return fail(WX_EARG, "n %d", n);
+  n = std::min(n, (int32_t)NPROF);
+  double mst[NPROF] = {};
+  int64_t ct[NPROF] = {};
+  rc = each_group(e, false, [&](int k) {
+    double m1[NPROF] = {};
+    int64_t c1[NPROF] = {};
+    int r = grp_profile(e->g[k], enable, m1, c1, n);
+    for (int j = 0; j < n; ++j) { mst[j] += m1[j]; ct[j] += c1[j]; }
+    return r;
+  });
+  for (int j = 0; j < n; ++j) {
+    if (ms) ms[j] = mst[j];
+    if (counts) counts[j] = ct[j];
+  }
+  return rc;
+}
+
+int wx_ens_info(wx_ens *e, int32_t *ninst, int32_t *nleaf_total, int32_t *grids, int32_t *threads) {
+  int rc = check_ens(e);
+  if (rc) return rc;
+  int32_t total = 0;
+  for (EnsGroup *g : e->g) total += g->nleaf_total;
+  if (ninst) *ninst = e->n;
+  if (nleaf_total) *nleaf_total = total;
+  return grp_info(e->g[0], nullptr, nullptr, grids, threads);
+}
+
+int wx_ens_groups(wx_ens *e, int32_t *groups) {
+  int rc = check_ens(e);
+  if (rc) return rc;
+  if (groups) *groups = (int32_t)e->g.size();
   return WX_OK;
 }
 

@@ -180,9 +180,9 @@ int fetch_flags(wx_ctx *c) {
 int do_admesh(wx_ctx *c, int *changed) {
   int rc = ensure_stats(c);
   if (rc) return rc;
-  HostTimer ht(c, HID_ADMESH);
   rc = fetch_flags(c);
   if (rc) return rc;
+  HostTimer ht(c, HID_ADMESH);   // host work only (after the step's wait)
   c->flag_of_leaf.resize(c->leaves.size());
   for (size_t l = 0; l < c->leaves.size(); ++l) c->flag_of_leaf[l] = c->hFlag[c->leaves[l].id];
   std::vector<MeshOp> ops;
