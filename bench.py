@@ -26,7 +26,8 @@ run's (the plain path's with --path plain); `plain`, `batched` and
   --no-surrogate    skip the batched+surrogate run
   --groups G        concurrently stepped groups of the ensemble (default 4)
   --no-tune --eager default launch shapes / no hipGraphs (for counter runs)
-  --no-cpu-baseline skip the CPU oracle timing
+  --no-cpu-baseline skip the CPU timings (the reference's own Fortran from
+                    oracle/_ref when built, and the C oracle)
 """
 from __future__ import annotations
 
@@ -49,6 +50,69 @@ L1_BOUND = 1e-3   # relative L1 density error the batched path is held to
 #   fillI_cool   read U^{n+1}, write the cooled U^{n+1}          2*neq*8
 # (the whole step's compulsory traffic is 5*neq*8: stages A and B).
 ALG_PASSES = {"stageA": 2, "stageB": 3, "fillI_cool": 2}
+
+
+def cpu_baseline_reference(cfg, seconds: float = 12.0):
+    """The reference's own Fortran (its hot-path sources built unmodified
+    into oracle/_ref/, one MPI rank, one core) stepping the same workload:
+    same initial mesh after the 4 initial admesh passes, then timestep +
+    tstep until ~`seconds` of CPU.  The reference's admesh is not part of
+    that build (its locatebounds/updatelpdown need an ifort extension), so
+    the mesh stays fixed and the list is installed once.  None when the
+    build is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from wxoracle import MeshState, Oracle, RefLib, ref_available
+    from walicxe2d_amd import params as P
+    variant = {"c1_explosion": "c1", "c2_sn_dmc": "c2", "c3_jet_coolingh": "c3"}.get(cfg.name)
+    if variant is None or not ref_available(variant):
+        return None
+    try:
+        R = RefLib(variant)
+    except OSError:
+        return None
+    O = Oracle()
+    pp = cfg.params()
+    m = MeshState(R.nblocks - 1, cfg.neq)
+    O.basegrid(pp, m)
+    u, lp, ic = P.initial_state(cfg)
+    n = lp.shape[0] - 1
+    m.u[1:n + 1] = u[1:]
+    for nb in range(1, n + 1):
+        m.primit[nb] = O.calcprim(pp, m.u[nb])
+    if cfg.nlevs > 1:
+        for _ in range(4):
+            O.admesh(pp, m)
+    O.locatebounds(pp, m)
+    R.setup(pp)
+    R.put(m)
+    leaves = len(m.leaves())
+    t = 0.0
+    steps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        dt = R.L.wxr_timestep()
+        t += dt
+        R.L.wxr_tstep(t, dt)
+        steps += 1
+    el = time.perf_counter() - t0
+    zones = float(steps) * leaves * 256
+    return {"value": zones / el, "unit": "zone-updates/s", "cores": 1, "kind": "reference",
+            "sample": f"{steps} steps from t=0 on the initial mesh ({leaves} leaves, fixed: the reference's admesh is "
+                      f"not in the build; {zones:.0f} zone-updates, {el:.1f} s), reference timestep + tstep compiled "
+                      "from its own sources with flang -O2, one MPI rank"}
+
+
+def _reference_in_child(workload: str):
+    """cpu_baseline_reference in a child process (the reference build
+    initialises MPI; a failure there must not end the bench)."""
+    import subprocess
+    try:
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-reference-only", "--workload", workload],
+                           capture_output=True, text=True, timeout=120)
+        lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+        return json.loads(lines[-1]) if r.returncode == 0 and lines else None
+    except (subprocess.TimeoutExpired, ValueError):
+        return None
 
 
 def cpu_baseline(cfg, seconds: float = 12.0):
@@ -194,7 +258,12 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes (one workgroup per block)")
     ap.add_argument("--eager", action="store_true", help="no hipGraphs: every launch from the host")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-reference-only", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_reference_only:
+        from walicxe2d_amd import params as P
+        print(json.dumps(cpu_baseline_reference(P.CONFIGS[args.workload]())), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs every leaf block on one "
               "GPU (multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)
@@ -278,7 +347,12 @@ def main():
     elif plain is not None:
         out["step_frac"] = plain["step_frac"]
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(P.CONFIGS[args.workload]())
+        # the reference's own code where its build is present, else the oracle
+        ref = _reference_in_child(args.workload)
+        port = cpu_baseline(P.CONFIGS[args.workload]())
+        out["cpu_baseline"] = ref if ref is not None else port
+        if ref is not None:
+            out["cpu_baseline_port"] = port
     print(json.dumps(out), flush=True)
 
 

@@ -193,3 +193,32 @@ def test_grouped_jet_ensemble_matches_ungrouped():
     assert a.info()[1] == b.info()[1]
     a.close()
     b.close()
+
+
+def test_c4_sixty_four_instances():
+    """The C4 workload: 64 explosion instances (max levels 3..6, seeded) in
+    one ensemble of 4 concurrently stepped groups, 6 steps with admesh;
+    instances from every group equal their single-mesh runs bit for bit."""
+    cfgs = [P.config_c4(i) for i in range(64)]
+    ens = Ensemble(cfgs, groups=4)
+    ens.basegrid()
+    for _ in range(3):
+        ens.admesh()
+    ens.set_graph(True)
+    picks = [0, 21, 42, 63]
+    sims = {}
+    for i in picks:
+        s = Simulation(cfgs[i])
+        s.basegrid()
+        for _ in range(3):
+            s.admesh()
+        sims[i] = s
+    for step in range(6):
+        ens.step(1)
+        for i in picks:
+            sims[i].step(1, admesh=cfgs[i].nlevs > 1)
+            _compare_instance(ens, i, sims[i], f"c4x64 step {step} instance {i}")
+    assert ens.info()[0] == 64
+    ens.close()
+    for s in sims.values():
+        s.close()
