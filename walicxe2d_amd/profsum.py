@@ -3,6 +3,9 @@ that `rocprofv3 -d DIR -o NAME` writes).
 
     python -m walicxe2d_amd.profsum kernels DB
         per-kernel calls, total / average duration, share, VGPRs, LDS, scratch
+    python -m walicxe2d_amd.profsum valu DB [DB ...]
+        per-wave SQ counters of `--pmc` runs of one command and the VALU
+        issue share of each kernel
     python -m walicxe2d_amd.profsum traffic --copy-fetch DB --copy-write DB \
             --fetch DB --write DB [--copy-mib 256]
         HBM bytes per zone-update of the hot-path kernels from two
@@ -47,6 +50,47 @@ def kernels(db: str) -> str:
     return "\n".join(out)
 
 
+def valu(dbs, clock_ghz: float = 2.4, simds: int = 1024) -> str:
+    """Per-kernel SQ counters per wave from one or more `--pmc` runs of the
+    same command (each run's counters divided by that run's SQ_WAVES), and
+    the VALU issue share: SQ_INSTS_VALU x 4 cycles (a wave64 instruction
+    takes 4 cycles on a 16-lane SIMD) over the SIMD-cycles of the launch
+    (duration x clock x SIMDs)."""
+    per = collections.defaultdict(dict)
+    dur = {}
+    for db in dbs:
+        cur = sqlite3.connect(db).cursor()
+        acc = collections.defaultdict(lambda: collections.defaultdict(float))
+        d = collections.defaultdict(lambda: [0.0, set()])
+        for name, did, cn, val, du in cur.execute(
+                "select kernel_name, dispatch_id, counter_name, value, duration from counters_collection"):
+            k = short(name)
+            acc[k][cn] += val
+            if did not in d[k][1]:
+                d[k][1].add(did)
+                d[k][0] += du
+        for k, c in acc.items():
+            w = c.get("SQ_WAVES", 0)
+            if not w:
+                continue
+            for cn, v in c.items():
+                if cn != "SQ_WAVES":
+                    per[k][cn] = v / w
+            per[k]["waves/launch"] = w / len(d[k][1])
+            if k not in dur:
+                dur[k] = d[k][0] / len(d[k][1])
+    out = ["| kernel | avg launch (us) | waves/launch | VALU/wave | VALU issue share | other counters per wave |",
+           "|---|---|---|---|---|---|"]
+    for k in sorted(per, key=lambda k: -dur.get(k, 0)):
+        c = per[k]
+        if "SQ_INSTS_VALU" not in c or k.startswith("__"):
+            continue
+        share = c["SQ_INSTS_VALU"] * c["waves/launch"] * 4 / (simds * dur[k] * 1e-9 * clock_ghz * 1e9)
+        rest = ", ".join(f"{n[3:]} {v:.0f}" for n, v in sorted(c.items()) if n not in ("SQ_INSTS_VALU", "waves/launch"))
+        out.append(f"| `{k}` | {dur[k] / 1e3:.2f} | {c['waves/launch']:.0f} | {c['SQ_INSTS_VALU']:.0f} | {share:.2f} | {rest} |")
+    return "\n".join(out)
+
+
 def _per_kernel(db: str):
     cur = sqlite3.connect(db).cursor()
     acc = collections.defaultdict(lambda: [0.0, 0, 0])
@@ -79,6 +123,8 @@ def main(argv=None):
     sub = ap.add_subparsers(dest="cmd", required=True)
     k = sub.add_parser("kernels")
     k.add_argument("db")
+    v = sub.add_parser("valu")
+    v.add_argument("dbs", nargs="+")
     t = sub.add_parser("traffic")
     t.add_argument("--copy-fetch", required=True)
     t.add_argument("--copy-write", required=True)
@@ -88,6 +134,8 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.cmd == "kernels":
         print(kernels(a.db))
+    elif a.cmd == "valu":
+        print(valu(a.dbs))
     else:
         print(json.dumps(traffic(a.copy_fetch, a.copy_write, a.fetch, a.write, a.copy_mib), indent=1))
     return 0
