@@ -122,11 +122,13 @@ __device__ __forceinline__ void ghost_coords(int side, int layer, int pos, int &
 // Flux sweep shared by both stages.  prim: LDS tile [NEQ][NC].  F: x-faces
 // fi=0..16 on rows 1..16 at [k][(j-1)*17+fi]; G: y-faces fj=0..16 on columns
 // 1..16 at [k][fj*16+(i-1)].  choice 2 reconstructs with the limiter.
-template <int NEQ, int CHOICE>
-__device__ __forceinline__ int flux_sweep(const wx_params *__restrict__ p, const double *prim,
-                                          double *F, double *G) {
+// SOLVER / LIM fix the solver and limiter at compile time (-1 / -99: read
+// from p); the arithmetic is the same either way.
+template <int NEQ, int CHOICE, int SOLVER = -1, int LIM = -99>
+__device__ __forceinline__ int flux_sweep_t(const wx_params *__restrict__ p, const double *prim,
+                                            double *F, double *G) {
   int err = 0;
-  const int lim = p->limiter;
+  const int lim = LIM != -99 ? LIM : p->limiter;
   for (int f = threadIdx.x; f < 544; f += blockDim.x) {
     // x-face fi on row j, or y-face fj on column i: the cells across the face
     // are `stride` apart, and a y-face's states are swapped into the x frame
@@ -158,7 +160,7 @@ __device__ __forceinline__ int flux_sweep(const wx_params *__restrict__ p, const
       }
       limiter<NEQ>(lim, qll, ql, qr, qrr);
     }
-    err |= riemann<NEQ>(p, ql, qr, ff);
+    err |= riemann<NEQ, SOLVER>(p, ql, qr, ff);
     if (!yf) {
       const int fi = f % 17, j = f / 17 + 1;
 #pragma unroll
@@ -171,6 +173,19 @@ __device__ __forceinline__ int flux_sweep(const wx_params *__restrict__ p, const
     }
   }
   return err;
+}
+
+// The sweep with the common configurations compiled in: HLLC, and in
+// stage B the minmod or van Leer limiter; anything else reads p.
+template <int NEQ, int CHOICE>
+__device__ __forceinline__ int flux_sweep(const wx_params *__restrict__ p, const double *prim,
+                                          double *F, double *G) {
+  if (p->solver == 0) {
+    if (CHOICE == 1) return flux_sweep_t<NEQ, CHOICE, 0>(p, prim, F, G);
+    if (p->limiter == 1) return flux_sweep_t<NEQ, CHOICE, 0, 1>(p, prim, F, G);
+    if (p->limiter == 2) return flux_sweep_t<NEQ, CHOICE, 0, 2>(p, prim, F, G);
+  }
+  return flux_sweep_t<NEQ, CHOICE>(p, prim, F, G);
 }
 
 // Cylindrical source terms, src/step.f90:49-76 (prims of the stage).
@@ -523,6 +538,19 @@ __device__ void cool_surrogate(const wx_params *__restrict__ p, SurDev *__restri
   }
 }
 
+// Cell c of the fill's per-cell passes: c < 256 the interior cell
+// (c % 16 + 1, c / 16 + 1); 256 <= c < 320 the layer-1 ghost that thread
+// c - 256 gathers (side (c - 256) / 16, position (c - 256) % 16 + 1).
+__device__ __forceinline__ void cool_cell(int c, int &i, int &j) {
+  if (c < 256) {
+    i = c % 16 + 1;
+    j = c / 16 + 1;
+  } else {
+    const int g = c - 256;
+    ghost_coords(g >> 4, 1, (g & 15) + 1, i, j);
+  }
+}
+
 template <int NEQ, bool SUR, int BS, bool PERSIST>
 __global__ __launch_bounds__(BS, (SUR ? 2 : 1024 / BS)) void k_fillI_cool(const wx_params *__restrict__ P, QueueRef q,
                                                     const int2 *__restrict__ ops,
@@ -582,12 +610,17 @@ __global__ __launch_bounds__(BS, (SUR ? 2 : 1024 / BS)) void k_fillI_cool(const 
       for (int k = 0; k < NEQ; ++k) tile[k * NC + IDX(gi, gj)] = g[k];
     }
   }
-  __syncthreads();
   // cooling over cells 0..n+1 (src/coolingh.f90:19-20, src/coolingdmc.f90:23-24)
+  // except the four corner cells, which nothing reads: the interior (cell
+  // c < 256) and the non-corner ghosts of layer 1 (c = 256 + the gather's
+  // index).  With 256 threads a ghost is cooled by the thread that gathered
+  // it, so the other waves start cooling the interior without waiting.
+  if (BS > 256) __syncthreads();
   const int mode = p->cooling;
   bool done = false;
   if constexpr (SUR && NEQ >= 5) {
     if (mode == 1) {
+      if (BS == 256) __syncthreads();
       cool_surrogate<NEQ>(p, M, W, S->dt * p->tsc, tile);
       __syncthreads();
       done = true;
@@ -595,8 +628,9 @@ __global__ __launch_bounds__(BS, (SUR ? 2 : 1024 / BS)) void k_fillI_cool(const 
   }
   if (!done && mode != 0) {
     const double dts = S->dt * p->tsc;
-    for (int c = tid; c < 18 * 18; c += blockDim.x) {
-      const int i = c % 18, j = c / 18;
+    for (int c = tid; c < 320; c += blockDim.x) {
+      int i, j;
+      cool_cell(c, i, j);
       double uu[NEQ];
 #pragma unroll
       for (int k = 0; k < NEQ; ++k) uu[k] = tile[k * NC + IDX(i, j)];
@@ -605,14 +639,16 @@ __global__ __launch_bounds__(BS, (SUR ? 2 : 1024 / BS)) void k_fillI_cool(const 
 #pragma unroll
       for (int k = 0; k < NEQ; ++k) tile[k * NC + IDX(i, j)] = uu[k];
     }
-    __syncthreads();
   }
+  __syncthreads();
   double *ob = Uout + (size_t)b * NEQ * NC;
   for (int c = tid; c < NEQ * NC; c += blockDim.x) ob[c] = tile[c];
   __syncthreads();
-  // primitives of cells 0..17 (non-corner use only) for the epilogue
-  for (int c = tid; c < 18 * 18; c += blockDim.x) {
-    const int i = c % 18, j = c / 18;
+  // primitives of the interior and the non-corner ghosts of layer 1 for the
+  // epilogue's 5-point stencils
+  for (int c = tid; c < 320; c += blockDim.x) {
+    int i, j;
+    cool_cell(c, i, j);
     double uu[NEQ], q[NEQ], T;
 #pragma unroll
     for (int k = 0; k < NEQ; ++k) uu[k] = tile[k * NC + IDX(i, j)];

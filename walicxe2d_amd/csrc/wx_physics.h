@@ -142,11 +142,13 @@ __device__ __forceinline__ void hllc_star(const wx_params *__restrict__ p, const
 // Riemann solvers: HLLC (src/hllcfluxes.f90:98-173), HLL
 // (src/hllfluxes.f90:98-131), HLL/HLLC hybrid (src/hll_hllcfluxes.f90:95-209).
 // Returns nonzero on the reference's `stop` path (NaN star speed).
-template <int NEQ>
+// SOLVER >= 0 fixes the solver at compile time (the same arithmetic, with
+// the other solvers' branches compiled out); -1 reads it from p.
+template <int NEQ, int SOLVER = -1>
 __device__ __forceinline__ int riemann(const wx_params *__restrict__ p, const double *pl,
                                        const double *pr, double *ff) {
   const double cv = p->cv, gam = p->gamma;
-  const int solver = p->solver;
+  const int solver = SOLVER >= 0 ? SOLVER : p->solver;
   double beta1 = 1.0, beta2 = 0.0;
   if (solver == 2) {
     double dq1 = pr[1] - pl[1], dq2 = pr[2] - pl[2];
