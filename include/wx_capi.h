@@ -37,6 +37,7 @@ extern "C" {
 #define WX_EARG      3   /* bad argument / inconsistent mesh */
 #define WX_EHIP      4   /* HIP runtime failure */
 #define WX_ENOMEM    5
+#define WX_EIO       6   /* checkpoint file missing, short or inconsistent */
 
 /* Layout selector for wx_upload / wx_download */
 #define WX_LAYOUT_BLOCK   0  /* [id][var][j][i], i fastest, id 0..cap (id 0 unused) */
@@ -157,6 +158,20 @@ int wx_profile(wx_ctx *ctx, int32_t enable, double *ms, int64_t *counts, int32_t
 
 /* Bookkeeping */
 int wx_info(wx_ctx *ctx, int32_t *nbmax, int32_t *nbleafs, int32_t *ninternal);
+/* neq, block-ID capacity and levels the context was created with. */
+int wx_shape(wx_ctx *ctx, int32_t *neq, int32_t *nblocks, int32_t *nlevs);
+
+/* BIN checkpoints.  wx_write_bin writes the reference's OUTBIN pair
+ * (src/output.f90:39-67): `pointers` = int32 nbmin=1, nbmax, nbleafs,
+ * nblocks; lp(1:nbmax,1:8), leaf(1:nbmax) as 4-byte logicals with .true. =
+ * logical_true (-1 as ifort writes it and py/walicxe2d_utils.py:48 expects,
+ * or 1 as flang writes it), icoord(1:nbmax,1:2), all column-major; `blocks` =
+ * u(nb,1:neq,-1:18,-1:18) of every leaf in ID order, FP64 little-endian.
+ * wx_read_bin is the warm start of src/basegrid.f90:167-251: it reads nfiles
+ * such pairs (one per rank of an MPI run, block IDs kept as written; the
+ * capacity must cover them) and replaces the context's mesh and state. */
+int wx_write_bin(wx_ctx *ctx, const char *pointers_path, const char *blocks_path, int32_t logical_true);
+int wx_read_bin(wx_ctx *ctx, int32_t nfiles, const char *const *pointers_paths, const char *const *blocks_paths);
 
 /*
  * Batched ensemble: n independent AMR meshes ("instances"), each with its own

@@ -20,6 +20,7 @@ extern "C" int wxsim_ops(int *out, int cap);
 extern "C" void wxsim_set_group_flags(int group, const signed char *f, int n);
 
 static int run_ensemble(int rounds, unsigned seed, int groups);
+static int bin_roundtrip(wx_ctx *c, const wx_params &p, const std::string &dir);
 
 int main(int argc, char **argv) {
   const int rounds = argc > 1 ? atoi(argv[1]) : 200;
@@ -91,6 +92,10 @@ int main(int argc, char **argv) {
     if (nint != wxo_locatebounds((const wxo_params *)&p, &m)) { fprintf(stderr, "round %d: list length differs\n", r); return 1; }
   }
   printf("hostsim: %d rounds, %d mesh ops, final nbmax %d leaves %d -- trees identical\n", rounds, total_ops, m.nbmax, m.nbleafs);
+  if (argc > 4 && std::string(argv[3]) == "bin") {
+    const int rc = bin_roundtrip(c, p, argv[4]);
+    if (rc) return rc;
+  }
   wx_destroy(c);
   return 0;
 }
@@ -204,5 +209,68 @@ static int run_ensemble(int rounds, unsigned seed, int groups) {
   printf("hostsim ensemble: %d instances in %d groups, %d rounds, %d mesh ops -- trees identical\n", n, groups, rounds,
          total_ops);
   wx_ens_destroy(e);
+  return 0;
+}
+
+// ---------------------------------------------------------------------
+// BIN checkpoints: the final mesh with a tile pattern (every slot, ghosts
+// included) is written by wx_write_bin, read back into a fresh context by
+// wx_read_bin and compared; the expected tree and tiles go to raw files
+// (expect_tree.i32: nbmax, nbleafs, lp(8 x nbmax), leaf, icoord(2 x
+// nbmax); expect_u.f64: (nbmax+1) tiles [var][j][i]) for the Python reader.
+static int bin_roundtrip(wx_ctx *c, const wx_params &p, const std::string &dir) {
+  int32_t nbmax = 0, nbleafs = 0;
+  wx_info(c, &nbmax, &nbleafs, nullptr);
+  const int ld = nbmax, neq = p.neq;
+  std::vector<int32_t> lp((size_t)8 * ld), leaf(ld), ic((size_t)2 * ld);
+  int32_t a = 0, b = 0;
+  if (wx_get_tree(c, lp.data(), leaf.data(), ic.data(), ld, &a, &b)) return 3;
+  std::vector<double> u((size_t)(nbmax + 1) * neq * 400);
+  for (size_t n = 0; n < u.size(); ++n) u[n] = 1.0 + 1e-3 * (double)(n % 100003) + (double)(n / 400) * 1e-9;
+  if (wx_upload(c, u.data(), WX_LAYOUT_BLOCK, lp.data(), leaf.data(), ic.data(), ld, nbmax, nbleafs)) return 3;
+  const std::string pp = dir + "/pointers000.001.bin", bp = dir + "/blocks000.001.bin";
+  if (wx_write_bin(c, pp.c_str(), bp.c_str(), -1)) {
+    char e[512];
+    wx_last_error(e, 512);
+    fprintf(stderr, "write_bin: %s\n", e);
+    return 3;
+  }
+  FILE *f = fopen((dir + "/expect_tree.i32").c_str(), "wb");
+  const int32_t head[2] = {nbmax, nbleafs};
+  fwrite(head, 4, 2, f);
+  fwrite(lp.data(), 4, lp.size(), f);
+  fwrite(leaf.data(), 4, leaf.size(), f);
+  fwrite(ic.data(), 4, ic.size(), f);
+  fclose(f);
+  f = fopen((dir + "/expect_u.f64").c_str(), "wb");
+  fwrite(u.data(), 8, u.size(), f);
+  fclose(f);
+  wx_ctx *c2 = nullptr;
+  if (wx_create(&p, &c2)) return 3;
+  const char *pa[1] = {pp.c_str()}, *ba[1] = {bp.c_str()};
+  if (wx_read_bin(c2, 1, pa, ba)) {
+    char e[512];
+    wx_last_error(e, 512);
+    fprintf(stderr, "read_bin: %s\n", e);
+    return 3;
+  }
+  std::vector<int32_t> lp2(lp.size()), leaf2(leaf.size()), ic2(ic.size());
+  if (wx_get_tree(c2, lp2.data(), leaf2.data(), ic2.data(), ld, &a, &b)) return 3;
+  if (a != nbmax || b != nbleafs || lp2 != lp || leaf2 != leaf || ic2 != ic) {
+    fprintf(stderr, "bin: tree differs after the round trip\n");
+    return 1;
+  }
+  std::vector<double> u2(u.size());
+  if (wx_download(c2, u2.data(), nullptr, WX_LAYOUT_BLOCK, 0)) return 3;
+  for (int nb = 1; nb <= nbmax; ++nb) {
+    if (!leaf[nb - 1]) continue;
+    for (size_t q = 0; q < (size_t)neq * 400; ++q)
+      if (u2[(size_t)nb * neq * 400 + q] != u[(size_t)nb * neq * 400 + q]) {
+        fprintf(stderr, "bin: tile of leaf %d differs at %zu\n", nb, q);
+        return 1;
+      }
+  }
+  wx_destroy(c2);
+  printf("bin: %d blocks, %d leaves written and read back identical\n", nbmax, nbleafs);
   return 0;
 }

@@ -1,0 +1,85 @@
+"""BIN checkpoints (src/output.f90:39-67, warm start src/basegrid.f90:
+167-251) on the CPU: the product's wx_write_bin / wx_read_bin built against
+the host-only HIP stand-in (tests/hostsim, ASan) write a refined mesh with a
+tile pattern and read it back into a fresh context; the Python 3 readers of
+walicxe2d_amd.binio (py/walicxe2d_utils.py restated) read the same files."""
+import os
+import shutil
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from walicxe2d_amd import binio
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "hostsim")
+pytestmark = pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not installed")
+
+
+@pytest.fixture(scope="module")
+def bindir(tmp_path_factory):
+    d = str(tmp_path_factory.mktemp("bin"))
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    r = subprocess.run([os.path.join(HERE, "hostsim_main"), "16", "3", "bin", d], capture_output=True, text=True,
+                       timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    assert "read back identical" in r.stdout
+    t = np.fromfile(os.path.join(d, "expect_tree.i32"), dtype=np.int32)
+    nbmax, nbleafs = int(t[0]), int(t[1])
+    lp = t[2:2 + 8 * nbmax].reshape(8, nbmax).T
+    leaf = t[2 + 8 * nbmax:2 + 9 * nbmax]
+    ic = t[2 + 9 * nbmax:2 + 11 * nbmax].reshape(2, nbmax).T
+    u = np.fromfile(os.path.join(d, "expect_u.f64")).reshape(nbmax + 1, 4, 20, 20)
+    return d, nbmax, nbleafs, lp, leaf, ic, u
+
+
+def test_pointers_file(bindir):
+    d, nbmax, nbleafs, lp, leaf, ic, _ = bindir
+    raw = np.fromfile(os.path.join(d, "pointers000.001.bin"), dtype="<i4")
+    assert raw.shape[0] == 4 + 11 * nbmax   # no record markers
+    assert list(raw[:4]) == [1, nbmax, nbleafs, 30000]
+    lg = raw[4 + 8 * nbmax:4 + 9 * nbmax]
+    assert set(np.unique(lg)) <= {0, -1} and (lg == -1).sum() == nbleafs   # ifort's .true.
+    nbmin, nbmax2, nbl, nblocks, lp2, leaf2, ic2 = binio.read_pointers(os.path.join(d, "pointers000.001.bin"))
+    assert (nbmin, nbmax2, nbl, nblocks) == (1, nbmax, nbleafs, 30000)
+    assert np.array_equal(lp2, lp) and np.array_equal(leaf2, leaf) and np.array_equal(ic2, ic)
+
+
+def test_blocks_file_and_reader(bindir):
+    d, nbmax, nbleafs, lp, leaf, ic, u = bindir
+    path = d + os.sep
+    size = os.path.getsize(os.path.join(d, "blocks000.001.bin"))
+    assert size == nbleafs * 4 * 400 * 8
+    blocks = binio.read_blocks(1, 16, 16, 5, 4, 1, 1.0, 1.0, 8, 8, path=path)
+    assert blocks.shape == (20, 20, 4, nbleafs)
+    ids = np.nonzero(leaf)[0] + 1
+    # [i, j, eq, leaf] of the reader against the tiles [id][var][j][i]
+    assert np.array_equal(blocks, u[ids].transpose(3, 2, 1, 0))
+    # the original reader's own loop order (outer y, then x, then the variable)
+    with open(os.path.join(d, "blocks000.001.bin"), "rb") as f:
+        first = np.zeros((20, 20, 4))
+        for i in range(20):
+            for j in range(20):
+                for q in range(4):
+                    first[j, i, q] = struct.unpack("d", f.read(8))[0]
+    assert np.array_equal(first, blocks[:, :, :, 0])
+
+
+def test_mesh_reader(bindir):
+    d, nbmax, nbleafs, lp, leaf, ic, _ = bindir
+    mesh = binio.read_mesh(1, 16, 16, 5, 1, 1.0, 1.0, 8, 8, path=d + os.sep)
+    assert mesh.shape == (nbleafs, 6)
+    ids = np.nonzero(leaf)[0]
+    lev = lp[ids, 0]
+    dx = 1.0 / 8 / 16 / 2.0 ** (lev - 1)
+    assert np.allclose(mesh[:, 0], ic[ids, 0] * dx) and np.allclose(mesh[:, 2], (ic[ids, 0] + 16) * dx)
+    assert np.array_equal(mesh[:, 4], lev)
+    # the leaves tile the unit square once
+    assert abs(((mesh[:, 2] - mesh[:, 0]) * (mesh[:, 3] - mesh[:, 1])).sum() - 1.0) < 1e-12
+
+
+def test_file_names():
+    assert binio.names("out", 7, 2) == (os.path.join("out", "BIN", "pointers002.007.bin"),
+                                         os.path.join("out", "BIN", "blocks002.007.bin"))
+    assert binio.names("out", 7, None)[0] == os.path.join("out", "BIN", "pointers007.bin")

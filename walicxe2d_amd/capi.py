@@ -31,6 +31,7 @@ EXPORTS = [
     "wx_ens_create", "wx_ens_destroy", "wx_ens_basegrid", "wx_ens_admesh", "wx_ens_step", "wx_ens_instance",
     "wx_ens_download", "wx_ens_get_tree", "wx_ens_set_graph", "wx_ens_set_launch", "wx_ens_tune",
     "wx_ens_profile", "wx_ens_info", "wx_ens_create_groups", "wx_ens_groups",
+    "wx_shape", "wx_write_bin", "wx_read_bin",
     "wx_sur_pairs", "wx_sur_features", "wx_sur_infer", "wx_sur_grad", "wx_sur_train", "wx_set_surrogate",
     "wx_ens_set_surrogate", "wx_sur_counters", "wx_ens_sur_counters",
 ]
@@ -87,6 +88,9 @@ def load_library(path: str = LIBPATH) -> ctypes.CDLL:
     L.wx_ens_create.argtypes = [ctypes.POINTER(WxParams), ctypes.c_int32, ctypes.POINTER(P)]
     L.wx_ens_create_groups.argtypes = [ctypes.POINTER(WxParams), ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(P)]
     L.wx_ens_groups.argtypes = [P, _ip]
+    L.wx_shape.argtypes = [P, _ip, _ip, _ip]
+    L.wx_write_bin.argtypes = [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32]
+    L.wx_read_bin.argtypes = [P, ctypes.c_int32, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p)]
     L.wx_ens_destroy.argtypes = [P]
     L.wx_ens_basegrid.argtypes = [P, ctypes.c_int32, _dp, ctypes.c_int32]
     L.wx_ens_admesh.argtypes = [P, _ip]
@@ -275,6 +279,25 @@ class Simulation:
         _check(self.L.wx_profile(self.h, enable, _d(ms), n.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 8),
                "wx_profile")
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(KERNELS)}
+
+    def set_time(self, t: float):
+        """Simulation time (e.g. itprint * dtprint after a warm start)."""
+        _check(self.L.wx_set_time(self.h, float(t)), "wx_set_time")
+        self.time = float(t)
+
+    def write_bin(self, pointers_path: str, blocks_path: str, logical_true: int = -1):
+        """The reference's OUTBIN checkpoint pair (src/output.f90:39-67)."""
+        _check(self.L.wx_write_bin(self.h, os.fsencode(pointers_path), os.fsencode(blocks_path), logical_true),
+               "wx_write_bin")
+
+    def read_bin(self, pointers_paths, blocks_paths):
+        """Warm start from one pair per rank (src/basegrid.f90:167-251)."""
+        n = len(pointers_paths)
+        if n != len(blocks_paths) or n < 1:
+            raise WxError("one blocks file per pointers file")
+        pp = (ctypes.c_char_p * n)(*[os.fsencode(x) for x in pointers_paths])
+        bp = (ctypes.c_char_p * n)(*[os.fsencode(x) for x in blocks_paths])
+        _check(self.L.wx_read_bin(self.h, n, pp, bp), "wx_read_bin")
 
     def set_surrogate(self, model):
         """Cooling surrogate (a walicxe2d_amd.surrogate.SurrogateModel) in the

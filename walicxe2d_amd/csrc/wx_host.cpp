@@ -488,6 +488,14 @@ int wx_sur_counters(wx_ctx *c, int32_t reset, int64_t *cells, int64_t *fallbacks
   return sur_counters(c->dSur.ptr, c->stream, reset, cells, fallbacks);
 }
 
+int wx_shape(wx_ctx *c, int32_t *neq, int32_t *nblocks, int32_t *nlevs) {
+  if (!c) return fail(WX_EARG, "null context");
+  if (neq) *neq = c->neq;
+  if (nblocks) *nblocks = c->cap;
+  if (nlevs) *nlevs = c->p.nlevs;
+  return WX_OK;
+}
+
 int wx_info(wx_ctx *c, int32_t *nbmax, int32_t *nbleafs, int32_t *ninternal) {
   if (!c) return fail(WX_EARG, "null context");
   if (nbmax) *nbmax = c->nbmax;
