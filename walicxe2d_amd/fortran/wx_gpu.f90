@@ -62,6 +62,14 @@ module wx_gpu
        real(c_double), intent(out) :: u(*), prim(*)
        integer(c_int32_t), value :: layout, ld
      end function wx_download
+     ! BIN checkpoints (src/output.f90:39-67 / src/basegrid.f90:167-251);
+     ! paths are NUL-terminated (trim(name)//c_null_char)
+     integer(c_int) function wx_write_bin(ctx, pointers, blocks, logical_true) bind(C, name="wx_write_bin")
+       import :: c_int, c_ptr, c_char, c_int32_t
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(in) :: pointers(*), blocks(*)
+       integer(c_int32_t), value :: logical_true
+     end function wx_write_bin
      integer(c_int) function wx_set_bounds(ctx, ib, ld, n) bind(C, name="wx_set_bounds")
        import :: c_int, c_ptr, c_int32_t
        type(c_ptr), value :: ctx
