@@ -26,6 +26,8 @@ run's (the plain path's with --path plain); `plain`, `batched` and
   --no-surrogate    skip the batched+surrogate run
   --groups G        concurrently stepped groups of the ensemble (default 4)
   --no-tune --eager default launch shapes / no hipGraphs (for counter runs)
+  --c4-steps N      steps of the 64-instance explosion ensemble (BASELINE
+                    config 4) against its instances run one by one (300)
   --no-cpu-baseline skip the CPU timings (the reference's own Fortran from
                     oracle/_ref when built, and the C oracle)
 """
@@ -199,6 +201,53 @@ def run_plain(cfg, steps, warmup, events=False):
     return zones / el, el * 1e3 / max(steps, 1), prof, zones, state, nleaf
 
 
+def run_c4(steps, warmup, groups):
+    """The batched explosion ensemble (64 DMC point explosions of 3..6
+    levels, seeded per instance): the ensemble in `groups` groups (tuned,
+    graph replays) against the 64 instances run one after the other on the
+    plain path, over the same steps; the first instances' final states are
+    compared bit for bit."""
+    import numpy as np
+    from walicxe2d_amd import params as P
+    from walicxe2d_amd.capi import Ensemble, Simulation
+    cfgs = [P.config_c4(i) for i in range(64)]
+    ens = Ensemble(cfgs, groups=groups)
+    ens.basegrid()
+    for _ in range(4):
+        ens.admesh()
+    if warmup:
+        ens.step(warmup)
+    ens.tune(reps=3)
+    ens.set_graph(True)
+    ens.step(0)
+    t0 = time.perf_counter()
+    zb = ens.step(steps)
+    eb = time.perf_counter() - t0
+    nleaf = ens.info()[1]
+    zp, ep, same = 0.0, 0.0, 0
+    for i, c in enumerate(cfgs):
+        sim = Simulation(c)
+        sim.basegrid()
+        for _ in range(4):
+            sim.admesh()
+        if warmup:
+            sim.step(warmup)
+        sim.step(0)
+        t0 = time.perf_counter()
+        _, _, z = sim.step(steps)
+        ep += time.perf_counter() - t0
+        zp += z
+        if i < 4:
+            leaf = sim.tree()[1]
+            ids = np.nonzero(leaf)[0]
+            same += int(np.array_equal(sim.download()[ids], ens.download(i)[ids]))
+        sim.close()
+    ens.close()
+    return {"workload": "c4_explosion_ensemble", "instances": 64, "steps": steps, "groups": groups,
+            "leaf_blocks_end_total": nleaf, "batched": zb / eb, "plain_sequential": zp / ep,
+            "speedup": (zb / eb) / (zp / ep), "instances_bit_identical": f"{same}/4"}
+
+
 def _final_state(sim):
     u = sim.download()
     lp, leaf, ic, _, _ = sim.tree()
@@ -257,6 +306,7 @@ def main():
     ap.add_argument("--groups", type=int, default=4, help="concurrently stepped groups of the ensemble")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes (one workgroup per block)")
     ap.add_argument("--eager", action="store_true", help="no hipGraphs: every launch from the host")
+    ap.add_argument("--c4-steps", type=int, default=300, help="steps of the 64-instance C4 comparison (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reference-only", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -346,6 +396,8 @@ def main():
             out["speedup_vs_plain"] = best["value"] / plain["value"]
     elif plain is not None:
         out["step_frac"] = plain["step_frac"]
+    if args.path == "both" and args.c4_steps > 0:
+        out["c4"] = run_c4(args.c4_steps, 20, args.groups)
     if not args.no_cpu_baseline:
         # the reference's own code where its build is present, else the oracle
         ref = _reference_in_child(args.workload)
