@@ -93,9 +93,9 @@ int launch_dtreduce(const wx_params *, const LeafRec *, const double *, const in
   }
   return 0;
 }
-int launch_gather_flags(QueueRef q, int, const int8_t *flag, int8_t *flagq, const DevScalars *S, DevScalars *Sout,
-                        int ninst, hipStream_t) {
-  for (int l = 0; l < q.hdr->nleaf; ++l) flagq[l] = flag[q.rec[l].id];
+int launch_gather_flags(QueueRef q, const int32_t *inst_off, int, const int8_t *flag, int8_t *flagq, const DevScalars *S,
+                        DevScalars *Sout, int ninst, hipStream_t) {
+  for (int l = 0; l < q.hdr->nleaf; ++l) flagq[inst_off[q.rec[l].inst] + q.rec[l].lidx] = flag[q.rec[l].id];
   if (Sout) for (int i = 0; i < ninst; ++i) Sout[i] = S[i];
   return 0;
 }

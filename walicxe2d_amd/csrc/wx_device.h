@@ -37,7 +37,7 @@ struct LeafRec {
   int32_t level;
   int32_t icx, icy;
   int32_t flags;   // bit0: block imposed by the jet source in this step's fills
-  int32_t lidx;    // position in the queue
+  int32_t lidx;    // position in the instance's leaf list (block-ID order)
   int32_t inst;    // instance: selects wx_params and DevScalars
   int32_t pad;
 };
@@ -109,9 +109,10 @@ int launch_blockstat(const wx_params *P, int neq, const LeafRec *leaves, int nle
 // [inst_off[i], inst_off[i+1]).
 int launch_dtreduce(const wx_params *P, const LeafRec *rec, const double *stat, const int32_t *inst_off,
                     int ninst, DevScalars *S, int advance_time, hipStream_t stream);
-// flagq[l] = flag[rec[l].id] for the queue; Sout[0..ninst) = S (if Sout).
-int launch_gather_flags(QueueRef q, int grid, const int8_t *flag, int8_t *flagq, const DevScalars *S,
-                        DevScalars *Sout, int ninst, hipStream_t stream);
+// flagq[inst_off[rec.inst] + rec.lidx] = flag[rec.id] over the queue (the
+// flags by leaf position, whatever order the queue holds); Sout = S.
+int launch_gather_flags(QueueRef q, const int32_t *inst_off, int grid, const int8_t *flag, int8_t *flagq,
+                        const DevScalars *S, DevScalars *Sout, int ninst, hipStream_t stream);
 int launch_meshops(int neq, const MeshOp *ops, int nops, double *U, hipStream_t stream);
 int launch_transpose(int neq, int cap, const double *src, double *dst, int to_fortran, int ld,
                      hipStream_t stream);

@@ -260,7 +260,8 @@ int launch_step(EnsGroup *e, int par, bool for_graph) {
   if (!rc) {
     const int g = for_graph ? std::max(1, (int)std::min<size_t>((e->nrec + 255) / 256, 1024))
                             : std::max(1, std::min((e->nleaf_total + 255) / 256, 1024));
-    rc = launch_gather_flags(q, g, e->dFlag.ptr, e->hFlagQ_dev, e->dS.ptr, e->hS_dev, e->n, e->stream);
+    rc = launch_gather_flags(q, e->dInstOff.ptr, g, e->dFlag.ptr, e->hFlagQ_dev, e->dS.ptr, e->hS_dev, e->n,
+                             e->stream);
   }
   if (rc) return fail(rc, "step launch failed: %s", hipGetErrorString(hipGetLastError()));
   return WX_OK;
@@ -438,7 +439,8 @@ int grp_admesh(EnsGroup *e, int32_t *changed) {
   if (!e) return fail(WX_EARG, "null ensemble");
   int rc = ensure_stats(e);
   if (rc) return rc;
-  rc = launch_gather_flags(e->queue(), std::max(1, std::min((e->nleaf_total + 255) / 256, 1024)), e->dFlag.ptr,
+  rc = launch_gather_flags(e->queue(), e->dInstOff.ptr, std::max(1, std::min((e->nleaf_total + 255) / 256, 1024)),
+                           e->dFlag.ptr,
                            e->hFlagQ_dev, e->dS.ptr, e->hS_dev, e->n, e->stream);
   if (rc) return fail(rc, "flag gather launch failed");
   int ch = 0;

@@ -24,6 +24,17 @@ namespace wx {
 
 __device__ __forceinline__ int IDX(int i, int j) { return (j + 1) * NP + (i + 1); }
 
+// Queue entry of workgroup b in a one-entry-per-workgroup launch of G
+// workgroups.  The hardware deals workgroups to the 8 XCDs round robin
+// (b % 8); within each run of 64 workgroups this hands XCD x the 8
+// consecutive entries 8x..8x+7, so that neighbouring blocks, which share
+// ghost data, are computed behind the same L2.  The last partial run keeps
+// the identity, so the map is a bijection of [0, G).
+__device__ __forceinline__ int xcd_entry(int b, int G) {
+  if (b >= (G & ~63)) return b;
+  return (b & ~63) | ((b & 7) << 3) | ((b >> 3) & 7);
+}
+
 // ---------------------------------------------------------------------
 // Ghost gather: value of one non-corner ghost cell of block `dst` at
 // (side, layer, pos) from the op that last wrote its segment.
@@ -214,11 +225,12 @@ __global__ __launch_bounds__(BS) void k_stageA(const wx_params *__restrict__ P, 
   __shared__ double F[NEQ * 272];
   __shared__ double G[NEQ * 272];
   // the first entry's record is read together with the queue length
-  LeafRec Lnext = q.rec[min((int)blockIdx.x, q.cap - 1)];
+  const int b0 = PERSIST ? (int)blockIdx.x : xcd_entry(blockIdx.x, gridDim.x);
+  LeafRec Lnext = q.rec[min(b0, q.cap - 1)];
   const int nleaf = q.hdr->nleaf;
   // one queue entry per workgroup, or (PERSIST) entries blockIdx.x +
   // k * gridDim.x; the loop form costs registers, so each is its own kernel
-  for (int l = blockIdx.x; l < nleaf; l += gridDim.x) {
+  for (int l = b0; l < nleaf; l += gridDim.x) {
   const LeafRec L = Lnext;
   if (PERSIST && l + (int)gridDim.x < nleaf) Lnext = q.rec[l + gridDim.x];
   const wx_params *__restrict__ p = P + L.inst;
@@ -269,9 +281,10 @@ __global__ __launch_bounds__(BS) void k_stageB(const wx_params *__restrict__ P, 
   __shared__ double prim[NEQ * NC];
   __shared__ double F[NEQ * 272];
   __shared__ double G[NEQ * 272];
-  LeafRec Lnext = q.rec[min((int)blockIdx.x, q.cap - 1)];
+  const int b0 = PERSIST ? (int)blockIdx.x : xcd_entry(blockIdx.x, gridDim.x);
+  LeafRec Lnext = q.rec[min(b0, q.cap - 1)];
   const int nleaf = q.hdr->nleaf;
-  for (int l = blockIdx.x; l < nleaf; l += gridDim.x) {
+  for (int l = b0; l < nleaf; l += gridDim.x) {
   const int r = l;
   const LeafRec L = Lnext;
   if (PERSIST && l + (int)gridDim.x < nleaf) Lnext = q.rec[l + gridDim.x];
@@ -570,9 +583,10 @@ __global__ __launch_bounds__(BS, (SUR ? 2 : 1024 / BS)) void k_fillI_cool(const 
     sur_stage(M, W);
     __syncthreads();
   }
-  LeafRec Lnext = q.rec[min((int)blockIdx.x, q.cap - 1)];
+  const int b0 = PERSIST ? (int)blockIdx.x : xcd_entry(blockIdx.x, gridDim.x);
+  LeafRec Lnext = q.rec[min(b0, q.cap - 1)];
   const int nleaf = q.hdr->nleaf;
-  for (int l = blockIdx.x; l < nleaf; l += gridDim.x) {
+  for (int l = b0; l < nleaf; l += gridDim.x) {
   const int r = l;
   const LeafRec L = Lnext;
   if (PERSIST && l + (int)gridDim.x < nleaf) Lnext = q.rec[l + gridDim.x];
@@ -754,12 +768,15 @@ __global__ __launch_bounds__(256) void k_build_queue(const QueueHdr *__restrict_
 // Refinement flags in queue order for the host's admesh, and the instances'
 // scalars; both outputs may be pinned host memory (mapped), so that the host
 // reads them after the stream synchronises with no copy launched.
-__global__ __launch_bounds__(256) void k_gather_flags(QueueRef q, const int8_t *__restrict__ flag,
+__global__ __launch_bounds__(256) void k_gather_flags(QueueRef q, const int32_t *__restrict__ inst_off,
+                                                      const int8_t *__restrict__ flag,
                                                       int8_t *__restrict__ flagq, const DevScalars *__restrict__ S,
                                                       DevScalars *__restrict__ Sout, int ninst) {
   const int n = q.hdr->nleaf;
-  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < n; l += gridDim.x * blockDim.x)
-    flagq[l] = flag[q.rec[l].id];
+  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < n; l += gridDim.x * blockDim.x) {
+    const LeafRec r = q.rec[l];
+    flagq[inst_off[r.inst] + r.lidx] = flag[r.id];   // by leaf position, whatever the queue order
+  }
   if (Sout && blockIdx.x == 0)
     for (int i = threadIdx.x; i < ninst; i += blockDim.x) Sout[i] = S[i];
 }
@@ -945,10 +962,11 @@ int launch_dtreduce(const wx_params *P, const LeafRec *rec, const double *stat, 
   return hip_status();
 }
 
-int launch_gather_flags(QueueRef q, int grid, const int8_t *flag, int8_t *flagq, const DevScalars *S,
+int launch_gather_flags(QueueRef q, const int32_t *inst_off, int grid, const int8_t *flag, int8_t *flagq,
+                        const DevScalars *S,
                         DevScalars *Sout, int ninst, hipStream_t stream) {
   if (grid <= 0) return WX_OK;
-  hipLaunchKernelGGL(k_gather_flags, dim3(grid), dim3(256), 0, stream, q, flag, flagq, S, Sout, ninst);
+  hipLaunchKernelGGL(k_gather_flags, dim3(grid), dim3(256), 0, stream, q, inst_off, flag, flagq, S, Sout, ninst);
   return hip_status();
 }
 
