@@ -26,8 +26,9 @@ run's (the plain path's with --path plain); `plain`, `batched` and
   --no-surrogate    skip the batched+surrogate run
   --groups G        concurrently stepped groups of the ensemble (default 4)
   --no-tune --eager default launch shapes / no hipGraphs (for counter runs)
-  --c4-steps N      steps of the 64-instance explosion ensemble (BASELINE
-                    config 4) against its instances run one by one (300)
+  --c4-steps N      steps of the 64-instance explosion ensemble (C4: DMC
+                    point explosions of 3..6 levels) against its instances
+                    run one by one (default 300, 0 skips it)
   --no-cpu-baseline skip the CPU timings (the reference's own Fortran from
                     oracle/_ref when built, and the C oracle)
 """

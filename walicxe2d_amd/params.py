@@ -103,7 +103,7 @@ def dmc_table(path: str | None = None) -> np.ndarray:
 
 @dataclass
 class Config:
-    """One problem set-up (a BASELINE config restated)."""
+    """One problem set-up: the workloads the path is measured and checked on."""
     name: str
     neq: int
     nlevs: int
@@ -192,7 +192,8 @@ class Config:
 
 
 # --------------------------------------------------------------------------
-# BASELINE configurations.
+# The workloads: C1 point explosion (adiabatic), C2 supernova with DMC
+# cooling and 5 levels, C3 cooling jet, C4 explosion ensemble, C5 jet variants.
 def config_c1(**kw) -> Config:
     """2-D point explosion, code units, 4x4 roots, AMR off, adiabatic."""
     c = Config(name="c1_explosion", neq=4, nlevs=1, nbrootx=4, nbrooty=4,
