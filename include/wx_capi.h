@@ -195,6 +195,11 @@ int wx_ens_create(const wx_params *p, int32_t n, wx_ens **out);
  * wx_ens_create is the one-group case. */
 int wx_ens_create_groups(const wx_params *p, int32_t n, int32_t groups, wx_ens **out);
 int wx_ens_groups(wx_ens *ens, int32_t *groups);
+/* neq, block-ID capacity and levels of instance inst. */
+int wx_ens_shape(wx_ens *ens, int32_t inst, int32_t *neq, int32_t *nblocks, int32_t *nlevs);
+/* The OUTBIN checkpoint pair of instance inst, as wx_write_bin. */
+int wx_ens_write_bin(wx_ens *ens, int32_t inst, const char *pointers_path, const char *blocks_path,
+                     int32_t logical_true);
 int wx_ens_destroy(wx_ens *ens);
 /* Root layout + initial data of instance inst (as wx_basegrid). */
 int wx_ens_basegrid(wx_ens *ens, int32_t inst, const double *uroots, int32_t nroots);

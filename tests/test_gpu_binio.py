@@ -41,3 +41,26 @@ def test_warm_start_continues_the_run(tmp_path):
         assert np.array_equal(ua[:, :, MASK], ub[:, :, MASK]), s
     a.close()
     b.close()
+
+
+def test_ensemble_instance_checkpoint_equals_single_run(tmp_path):
+    """wx_ens_write_bin of an instance writes the same bytes as wx_write_bin
+    of the same mesh run alone (the ensemble is bit-identical to it)."""
+    from walicxe2d_amd.capi import Ensemble
+    cfgs = [P.config_c4(i, nblocks=3000) for i in (0, 9)]
+    e = Ensemble(cfgs, groups=2)
+    e.basegrid()
+    for _ in range(3):
+        e.admesh()
+    e.step(5)
+    s = Simulation(cfgs[1])
+    s.basegrid()
+    for _ in range(3):
+        s.admesh()
+    s.step(5)
+    e.write_bin(1, str(tmp_path / "pe.bin"), str(tmp_path / "be.bin"))
+    s.write_bin(str(tmp_path / "ps.bin"), str(tmp_path / "bs.bin"))
+    assert (tmp_path / "pe.bin").read_bytes() == (tmp_path / "ps.bin").read_bytes()
+    assert (tmp_path / "be.bin").read_bytes() == (tmp_path / "bs.bin").read_bytes()
+    e.close()
+    s.close()

@@ -5,8 +5,9 @@ struct.unpack there), here for Python 3 with numpy.
 File names follow the reference: with MPI (`src/output.f90:44-45`)
 `<path>/BIN/pointers<rank:03>.<itprint:03>.bin` and `blocks...`, without
 MPI `pointers<itprint:03>.bin`.  The files themselves are written and read
-by the library (`wx_write_bin` / `wx_read_bin`): Simulation.write_bin /
-read_bin, or `save` / `warm_start` below.
+by the library (`wx_write_bin` / `wx_read_bin`, and `wx_ens_write_bin` for
+one instance of an ensemble): Simulation.write_bin / read_bin,
+Ensemble.write_bin, or `save` / `warm_start` below.
 """
 from __future__ import annotations
 

@@ -31,7 +31,7 @@ EXPORTS = [
     "wx_ens_create", "wx_ens_destroy", "wx_ens_basegrid", "wx_ens_admesh", "wx_ens_step", "wx_ens_instance",
     "wx_ens_download", "wx_ens_get_tree", "wx_ens_set_graph", "wx_ens_set_launch", "wx_ens_tune",
     "wx_ens_profile", "wx_ens_info", "wx_ens_create_groups", "wx_ens_groups",
-    "wx_shape", "wx_write_bin", "wx_read_bin",
+    "wx_shape", "wx_write_bin", "wx_read_bin", "wx_ens_shape", "wx_ens_write_bin",
     "wx_sur_pairs", "wx_sur_features", "wx_sur_infer", "wx_sur_grad", "wx_sur_train", "wx_set_surrogate",
     "wx_ens_set_surrogate", "wx_sur_counters", "wx_ens_sur_counters",
 ]
@@ -91,6 +91,8 @@ def load_library(path: str = LIBPATH) -> ctypes.CDLL:
     L.wx_shape.argtypes = [P, _ip, _ip, _ip]
     L.wx_write_bin.argtypes = [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32]
     L.wx_read_bin.argtypes = [P, ctypes.c_int32, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p)]
+    L.wx_ens_shape.argtypes = [P, ctypes.c_int32, _ip, _ip, _ip]
+    L.wx_ens_write_bin.argtypes = [P, ctypes.c_int32, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32]
     L.wx_ens_destroy.argtypes = [P]
     L.wx_ens_basegrid.argtypes = [P, ctypes.c_int32, _dp, ctypes.c_int32]
     L.wx_ens_admesh.argtypes = [P, _ip]
@@ -421,6 +423,11 @@ class Ensemble:
         t = np.zeros(3, dtype=np.int32)
         _check(self.L.wx_ens_info(self.h, ctypes.byref(a), ctypes.byref(b), _i(g), _i(t)), "wx_ens_info")
         return a.value, b.value, [int(x) for x in g], [int(x) for x in t]
+
+    def write_bin(self, i: int, pointers_path: str, blocks_path: str, logical_true: int = -1):
+        """The reference's OUTBIN checkpoint pair of instance i."""
+        _check(self.L.wx_ens_write_bin(self.h, i, os.fsencode(pointers_path), os.fsencode(blocks_path),
+                                       logical_true), "wx_ens_write_bin")
 
     def set_surrogate(self, model):
         _check(self.L.wx_ens_set_surrogate(self.h, ctypes.byref(model.to_c()) if model is not None else None),

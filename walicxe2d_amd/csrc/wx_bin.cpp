@@ -32,22 +32,16 @@ constexpr int NPAD = NP;   // 20 cells per padded row
 
 }  // namespace
 
-extern "C" {
 
-int wx_write_bin(wx_ctx *c, const char *pointers_path, const char *blocks_path, int32_t logical_true) {
-  if (!c || !pointers_path || !blocks_path) return fail(WX_EARG, "null argument");
+namespace {
+
+// The OUTBIN pair of one mesh: tree arrays in Fortran layout (ld = nbmax)
+// and the tiles [id][var][j][i] of IDs 0..nbmax.
+int write_pair(const char *pointers_path, const char *blocks_path, int32_t nbmax, int32_t nbleafs, int32_t cap,
+               int neq, const std::vector<int32_t> &lp, const std::vector<int32_t> &leaf,
+               const std::vector<int32_t> &ic, const std::vector<double> &u, int32_t logical_true) {
   if (logical_true != -1 && logical_true != 1) return fail(WX_EARG, "logical_true %d (-1 ifort, 1 flang)", logical_true);
-  int32_t neq = 0, cap = 0, nbmax = 0, nbleafs = 0;
-  int rc = wx_shape(c, &neq, &cap, nullptr);
-  if (!rc) rc = wx_info(c, &nbmax, &nbleafs, nullptr);
-  if (rc) return rc;
-  if (nbmax < 1) return fail(WX_EARG, "no mesh to write");
   const int ld = nbmax;
-  std::vector<int32_t> lp((size_t)8 * ld), leaf(ld), ic((size_t)2 * ld);
-  int32_t nbm = 0, nbl = 0;
-  if ((rc = wx_get_tree(c, lp.data(), leaf.data(), ic.data(), ld, &nbm, &nbl))) return rc;
-  std::vector<double> u((size_t)(nbmax + 1) * neq * NC);
-  if ((rc = wx_download(c, u.data(), nullptr, WX_LAYOUT_BLOCK, 0))) return rc;
   {
     File out;
     if (!(out.f = fopen(pointers_path, "wb"))) return fail(WX_EIO, "cannot open %s", pointers_path);
@@ -71,6 +65,43 @@ int wx_write_bin(wx_ctx *c, const char *pointers_path, const char *blocks_path, 
     if (fwrite(blk.data(), 8, blk.size(), out.f) != blk.size()) return fail(WX_EIO, "short write to %s", blocks_path);
   }
   return WX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wx_write_bin(wx_ctx *c, const char *pointers_path, const char *blocks_path, int32_t logical_true) {
+  if (!c || !pointers_path || !blocks_path) return fail(WX_EARG, "null argument");
+  int32_t neq = 0, cap = 0, nbmax = 0, nbleafs = 0;
+  int rc = wx_shape(c, &neq, &cap, nullptr);
+  if (!rc) rc = wx_info(c, &nbmax, &nbleafs, nullptr);
+  if (rc) return rc;
+  if (nbmax < 1) return fail(WX_EARG, "no mesh to write");
+  const int ld = nbmax;
+  std::vector<int32_t> lp((size_t)8 * ld), leaf(ld), ic((size_t)2 * ld);
+  int32_t nbm = 0, nbl = 0;
+  if ((rc = wx_get_tree(c, lp.data(), leaf.data(), ic.data(), ld, &nbm, &nbl))) return rc;
+  std::vector<double> u((size_t)(nbmax + 1) * neq * NC);
+  if ((rc = wx_download(c, u.data(), nullptr, WX_LAYOUT_BLOCK, 0))) return rc;
+  return write_pair(pointers_path, blocks_path, nbmax, nbleafs, cap, neq, lp, leaf, ic, u, logical_true);
+}
+
+int wx_ens_write_bin(wx_ens *e, int32_t inst, const char *pointers_path, const char *blocks_path,
+                     int32_t logical_true) {
+  if (!e || !pointers_path || !blocks_path) return fail(WX_EARG, "null argument");
+  int32_t neq = 0, cap = 0, nbmax = 0, nbleafs = 0;
+  int rc = wx_ens_shape(e, inst, &neq, &cap, nullptr);
+  if (!rc) rc = wx_ens_instance(e, inst, nullptr, nullptr, &nbmax, &nbleafs);
+  if (rc) return rc;
+  if (nbmax < 1) return fail(WX_EARG, "instance %d has no mesh to write", inst);
+  const int ld = nbmax;
+  std::vector<int32_t> lp((size_t)8 * ld), leaf(ld), ic((size_t)2 * ld);
+  int32_t nbm = 0, nbl = 0;
+  if ((rc = wx_ens_get_tree(e, inst, lp.data(), leaf.data(), ic.data(), ld, &nbm, &nbl))) return rc;
+  std::vector<double> u((size_t)(nbmax + 1) * neq * NC);
+  if ((rc = wx_ens_download(e, inst, u.data()))) return rc;
+  return write_pair(pointers_path, blocks_path, nbmax, nbleafs, cap, neq, lp, leaf, ic, u, logical_true);
 }
 
 int wx_read_bin(wx_ctx *c, int32_t nfiles, const char *const *pointers_paths, const char *const *blocks_paths) {

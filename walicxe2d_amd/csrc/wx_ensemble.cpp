@@ -864,6 +864,16 @@ int wx_ens_info(wx_ens *e, int32_t *ninst, int32_t *nleaf_total, int32_t *grids,
   return grp_info(e->g[0], nullptr, nullptr, grids, threads);
 }
 
+int wx_ens_shape(wx_ens *e, int32_t inst, int32_t *neq, int32_t *nblocks, int32_t *nlevs) {
+  int rc = check_ens(e, inst);
+  if (rc) return rc;
+  const Mesh &m = e->g[e->gi[inst]]->m[e->li[inst]];
+  if (neq) *neq = m.p.neq;
+  if (nblocks) *nblocks = m.cap;
+  if (nlevs) *nlevs = m.p.nlevs;
+  return WX_OK;
+}
+
 int wx_ens_groups(wx_ens *e, int32_t *groups) {
   int rc = check_ens(e);
   if (rc) return rc;
