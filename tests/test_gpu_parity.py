@@ -13,8 +13,11 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12   # relative, per cell per step
 # coolingh's neutral-fraction update (src/coolingh.f90:218-228) moves y by up
 # to 1.1e-12 (relative to rho) for a one-ulp change of exp() (measured over
-# 2e5 random states); the device libm and glibc differ by an ulp at times, so
-# the n_HI scalar under COOLINGH is held to ten times that.
+# 2e5 random states), and a 1e-15 relative change of T moves the cooled
+# energy by up to 1.9e-12 and y by up to 3.8e-12 (oracle's atomic, 2e4
+# random states); the device libm and glibc differ by an ulp at times (exp,
+# pow, and sin in the jet's velocity), so under COOLINGH the energy and the
+# n_HI scalar are held to 1e-11.
 TOL_HI = 1e-11
 
 
@@ -43,13 +46,14 @@ def _pair_run(cfg, nsteps, cap, admesh, tol=TOL, label=""):
         assert abs(tg - t) <= TOL * t
         ug = sim.download()
         lv = m.leaves()
-        e, where = rel_err(ug[lv][:, :4], m.u[lv][:, :4], MASK)
+        nh = 3 if cfg.cooling == "H" else 4   # variables held to tol
+        e, where = rel_err(ug[lv][:, :nh], m.u[lv][:, :nh], MASK)
         worst = max(worst, e)
         assert e <= tol, f"{label} step {s}: rel err {e:.3e} at var/leaf/cell {where}"
-        if cfg.neq > 4:
+        if cfg.neq > 4 or cfg.cooling == "H":
             e5, where5 = rel_err(ug[lv], m.u[lv], MASK)
             lim = TOL_HI if cfg.cooling == "H" else tol
-            assert e5 <= lim, f"{label} step {s}: scalar rel err {e5:.3e} at {where5}"
+            assert e5 <= lim, f"{label} step {s}: energy/scalar rel err {e5:.3e} at {where5}"
         if admesh and cfg.nlevs > 1:
             assert O.admesh(pp, m) == 0
             sim.admesh()
@@ -78,6 +82,15 @@ def test_c3_jet_coolingh_amr_steps():
     cfg = P.config_c3(nblocks=6000, nbrootx=16, nbrooty=4)
     w = _pair_run(cfg, 25, 6000, admesh=True, label="c3")
     print("c3 worst rel err", w)
+
+
+def test_c3_jet_full_size_steps():
+    """The cooling jet at its full size (64x16 roots, 4 levels, the jet
+    source, coolingh, reflective left/bottom), admesh every step on both
+    sides, 20 steps: trees identical and cells within the per-step bound."""
+    cfg = P.config_c3(nblocks=8000)
+    w = _pair_run(cfg, 20, 8000, admesh=True, label="c3 full size")
+    print("c3 full-size worst rel err", w)
 
 
 def _variant(**kw):

@@ -23,11 +23,16 @@ struct wx_ctx : Mesh, Runtime {
   int cur = 0;   // U[cur] holds the current state
   DevBuf<wx_params> dP;
   DevBuf<DevScalars> dS;
-  DevBuf<QueueHdr> dQ;
-  DevBuf<int32_t> dInstOff;
-  DevBuf<LeafRec> dLeaves;
-  DevBuf<BlockRec> dBinfo;
-  DevBuf<int2> dOpsII, dOpsI;
+  // the tables of the current mesh in one device arena, uploaded by one
+  // copy per mesh change: queue header, instance offsets {0, nleaf}, leaf
+  // records, ghost ops of fills II and I, block records
+  DevBuf<char> dTab;
+  QueueHdr *tQ = nullptr;
+  int32_t *tOff = nullptr;
+  LeafRec *tLeaves = nullptr;
+  int2 *tOpsII = nullptr, *tOpsI = nullptr;
+  BlockRec *tBinfo = nullptr;
+  int32_t tLeafCap = 1;
   DevBuf<double> dStat;
   DevBuf<int8_t> dFlag;
   DevBuf<MeshOp> dMeshOps;
@@ -35,10 +40,11 @@ struct wx_ctx : Mesh, Runtime {
   std::vector<LeafRec> dirty;
   DevBuf<LeafRec> dDirty;
   // pinned staging of the device scalars and the flags
-  DevScalars *hS = nullptr;
+  DevScalars *hS = nullptr, *hS_dev = nullptr;   // mapped: k_gather_flags writes it
   int8_t *hFlag = nullptr;
   size_t hFlagN = 0;
-  std::vector<int8_t> flag_of_leaf;
+  // the flags by leaf position, written by k_gather_flags into mapped memory
+  int8_t *hFlagQ = nullptr, *hFlagQ_dev = nullptr;
   // cooling surrogate (coolingh only), used when use_sur
   DevBuf<SurDev> dSur;
   bool use_sur = false;
@@ -46,28 +52,39 @@ struct wx_ctx : Mesh, Runtime {
   double *cur_u() { return U[cur].ptr; }
   double *other_u() { return U[cur ^ 1].ptr; }
   int nleaf() const { return (int)leaves.size(); }
-  QueueRef queue() const { return QueueRef{dQ.ptr, dLeaves.ptr, (int32_t)dLeaves.n}; }
+  QueueRef queue() const { return QueueRef{tQ, tLeaves, tLeafCap}; }
 };
 
 namespace {
 
 int upload_tables(wx_ctx *c) {
   const size_t nleaf = c->leaves.size();
+  const size_t bl = nleaf * sizeof(LeafRec), bo = nleaf * 8 * sizeof(int2), bb = c->binfo.size() * sizeof(BlockRec);
+  const size_t total = 2 * 256 + al256(bl) + 2 * al256(bo) + al256(bb);
   int rc;
-  if ((rc = c->dLeaves.ensure(nleaf))) return rc;
-  if ((rc = c->dBinfo.ensure(c->binfo.size()))) return rc;
-  if ((rc = c->dOpsII.ensure(nleaf * 8))) return rc;
-  if ((rc = c->dOpsI.ensure(nleaf * 8))) return rc;
+  if (total > c->dTab.n) {   // grow with room for the next mesh changes
+    if ((rc = c->dTab.ensure(total + total / 2))) return rc;
+  }
+  char *base = c->dTab.ptr;
+  c->tQ = (QueueHdr *)base;
+  c->tOff = (int32_t *)(base + 256);
+  c->tLeaves = (LeafRec *)(base + 512);
+  c->tOpsII = (int2 *)((char *)c->tLeaves + al256(bl));
+  c->tOpsI = (int2 *)((char *)c->tOpsII + al256(bo));
+  c->tBinfo = (BlockRec *)((char *)c->tOpsI + al256(bo));
+  c->tLeafCap = (int32_t)std::max<size_t>(nleaf, 1);
+  if ((rc = stage_begin(c, total))) return rc;
+  char *h = stage_take(c, total);
+  if (!h) return fail(WX_EARG, "staging %zu bytes", total);
   const QueueHdr qh{(int32_t)nleaf, 1, {0, 0}};
   const int32_t off[2] = {0, (int32_t)nleaf};
-  const size_t bl = nleaf * sizeof(LeafRec), bo = nleaf * 8 * sizeof(int2), bb = c->binfo.size() * sizeof(BlockRec);
-  if ((rc = stage_begin(c, al256(bl) + 2 * al256(bo) + al256(bb) + 2 * 256))) return rc;
-  if ((rc = stage_copy(c, c->dLeaves.ptr, c->leaves.data(), bl))) return rc;
-  if ((rc = stage_copy(c, c->dOpsII.ptr, c->opsII.data(), bo))) return rc;
-  if ((rc = stage_copy(c, c->dOpsI.ptr, c->opsI.data(), bo))) return rc;
-  if ((rc = stage_copy(c, c->dBinfo.ptr, c->binfo.data(), bb))) return rc;
-  if ((rc = stage_copy(c, c->dQ.ptr, &qh, sizeof qh))) return rc;
-  if ((rc = stage_copy(c, c->dInstOff.ptr, off, sizeof off))) return rc;
+  memcpy(h, &qh, sizeof qh);
+  memcpy(h + 256, off, sizeof off);
+  memcpy(h + 512, c->leaves.data(), bl);
+  memcpy(h + 512 + al256(bl), c->opsII.data(), bo);
+  memcpy(h + 512 + al256(bl) + al256(bo), c->opsI.data(), bo);
+  memcpy(h + 512 + al256(bl) + 2 * al256(bo), c->binfo.data(), bb);
+  if ((rc = stage_push(c, base, h, total))) return rc;
   stage_end(c);
   return WX_OK;
 }
@@ -85,7 +102,7 @@ int ensure_stats(wx_ctx *c) {
   if (rc) return rc;
   if (c->stats_all_stale) {
     ProfScope ps(c, KID_STAT);
-    rc = launch_blockstat(c->dP.ptr, c->neq, c->dLeaves.ptr, c->nleaf(), c->cur_u(), c->dStat.ptr,
+    rc = launch_blockstat(c->dP.ptr, c->neq, c->tLeaves, c->nleaf(), c->cur_u(), c->dStat.ptr,
                           c->dFlag.ptr, c->stream);
     if (rc) return fail(rc, "blockstat launch failed");
     c->stats_all_stale = false;
@@ -128,16 +145,16 @@ int run_stages(wx_ctx *c) {
   }
   if (!rc) {
     ProfScope ps(c, KID_STAGEB);
-    rc = launch_stageB(c->dP.ptr, c->neq, c->queue(), g, 256, 0, c->dOpsII.ptr, c->dBinfo.ptr, c->cur_u(), c->other_u(),
+    rc = launch_stageB(c->dP.ptr, c->neq, c->queue(), g, 256, 0, c->tOpsII, c->tBinfo, c->cur_u(), c->other_u(),
                        c->dS.ptr, c->stream);
   }
   if (!rc) {
     ProfScope ps(c, KID_FILLI);
     if (c->use_sur)
-      rc = launch_fillI_cool_sur(c->dP.ptr, c->neq, c->queue(), g, 256, 0, c->dOpsI.ptr, c->dBinfo.ptr, c->cur_u(),
+      rc = launch_fillI_cool_sur(c->dP.ptr, c->neq, c->queue(), g, 256, 0, c->tOpsI, c->tBinfo, c->cur_u(),
                                  c->other_u(), c->dStat.ptr, c->dFlag.ptr, c->dS.ptr, c->dSur.ptr, c->stream);
     else
-      rc = launch_fillI_cool(c->dP.ptr, c->neq, c->queue(), g, 256, 0, c->dOpsI.ptr, c->dBinfo.ptr, c->cur_u(),
+      rc = launch_fillI_cool(c->dP.ptr, c->neq, c->queue(), g, 256, 0, c->tOpsI, c->tBinfo, c->cur_u(),
                              c->other_u(), c->dStat.ptr, c->dFlag.ptr, c->dS.ptr, c->stream);
   }
   if (rc) return fail(rc, "stage launch failed: %s", hipGetErrorString(hipGetLastError()));
@@ -177,16 +194,30 @@ int fetch_flags(wx_ctx *c) {
   return sync_scalars(c);
 }
 
+// The flags of the leaves by leaf position and the device scalars, written
+// by one kernel into mapped host memory; then the step's wait and the
+// Riemann-failure check of sync_scalars.
+int fetch_leaf_flags(wx_ctx *c) {
+  const int n = c->nleaf();
+  int rc = launch_gather_flags(c->queue(), c->tOff, std::max(1, std::min((n + 255) / 256, 1024)), c->dFlag.ptr,
+                               c->hFlagQ_dev, c->dS.ptr, c->hS_dev, 1, c->stream);
+  if (rc) return fail(rc, "flag gather launch failed");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->hS->err) {
+    int e = c->hS->err;
+    return fail(WX_EHLLC, "Error in hllc (device flag %d): a Riemann problem had no valid star speed", e);
+  }
+  return WX_OK;
+}
+
 int do_admesh(wx_ctx *c, int *changed) {
   int rc = ensure_stats(c);
   if (rc) return rc;
-  rc = fetch_flags(c);
+  rc = fetch_leaf_flags(c);
   if (rc) return rc;
   HostTimer ht(c, HID_ADMESH);   // host work only (after the step's wait)
-  c->flag_of_leaf.resize(c->leaves.size());
-  for (size_t l = 0; l < c->leaves.size(); ++l) c->flag_of_leaf[l] = c->hFlag[c->leaves[l].id];
   std::vector<MeshOp> ops;
-  rc = admesh_from_flags(c, c->flag_of_leaf.data(), ops);
+  rc = admesh_from_flags(c, c->hFlagQ, ops);
   if (rc) return rc;
   if (changed) *changed = (int)ops.size();
   return run_meshops(c, ops);
@@ -217,18 +248,32 @@ int wx_create(const wx_params *p, wx_ctx **out) {
   for (int k = 0; k < 2 && !rc; ++k) rc = c->U[k].ensure(tiles);
   if (!rc) rc = c->dP.ensure(1);
   if (!rc) rc = c->dS.ensure(1);
-  if (!rc) rc = c->dQ.ensure(1);
-  if (!rc) rc = c->dInstOff.ensure(2);
+  if (!rc) rc = c->dTab.ensure(4096);   // an empty queue until the first tables
   if (!rc) rc = c->dStat.ensure((size_t)(c->cap + 1) * 2);
   if (!rc) rc = c->dFlag.ensure((size_t)c->cap + 1);
   if (rc) { wx_destroy(c); return rc; }
+  c->tQ = (QueueHdr *)c->dTab.ptr;
+  c->tOff = (int32_t *)(c->dTab.ptr + 256);
+  c->tLeaves = (LeafRec *)(c->dTab.ptr + 512);
+  c->tOpsII = c->tOpsI = (int2 *)(c->dTab.ptr + 1024);
+  c->tBinfo = (BlockRec *)(c->dTab.ptr + 2048);
   for (int k = 0; k < 2; ++k) (void)hipMemsetAsync(c->U[k].ptr, 0, tiles * sizeof(double), c->stream);
-  (void)hipMemsetAsync(c->dQ.ptr, 0, sizeof(QueueHdr), c->stream);
-  (void)hipMemsetAsync(c->dInstOff.ptr, 0, 2 * sizeof(int32_t), c->stream);
+  (void)hipMemsetAsync(c->dTab.ptr, 0, 4096, c->stream);
   (void)hipMemcpyAsync(c->dP.ptr, p, sizeof(wx_params), hipMemcpyHostToDevice, c->stream);
   DevScalars z{};
   (void)hipMemcpyAsync(c->dS.ptr, &z, sizeof z, hipMemcpyHostToDevice, c->stream);
-  if (hipHostMalloc((void **)&c->hS, sizeof(DevScalars)) != hipSuccess) { wx_destroy(c); return fail(WX_EHIP, "pinned alloc"); }
+  if (hipHostMalloc((void **)&c->hS, sizeof(DevScalars), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+      hipHostMalloc((void **)&c->hFlagQ, (size_t)c->cap + 1, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+    wx_destroy(c);
+    return fail(WX_EHIP, "pinned alloc");
+  }
+  if (hipHostGetDevicePointer((void **)&c->hS_dev, c->hS, 0) != hipSuccess ||
+      hipHostGetDevicePointer((void **)&c->hFlagQ_dev, c->hFlagQ, 0) != hipSuccess) {
+    wx_destroy(c);
+    return fail(WX_EHIP, "mapped pointers of the pinned mirrors");
+  }
+  memset(c->hS, 0, sizeof(DevScalars));
+  memset(c->hFlagQ, 0, (size_t)c->cap + 1);
   if (hipStreamSynchronize(c->stream) != hipSuccess) { wx_destroy(c); return fail(WX_EHIP, "init sync"); }
   *out = c;
   return WX_OK;
@@ -238,12 +283,12 @@ int wx_destroy(wx_ctx *c) {
   if (!c) return WX_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto &b : c->U) b.release();
-  c->dP.release(); c->dS.release(); c->dQ.release(); c->dInstOff.release(); c->dLeaves.release();
-  c->dBinfo.release(); c->dOpsII.release(); c->dOpsI.release(); c->dStat.release(); c->dFlag.release();
+  c->dP.release(); c->dS.release(); c->dTab.release(); c->dStat.release(); c->dFlag.release();
   c->dMeshOps.release(); c->dScratch.release(); c->dDirty.release(); c->dSur.release();
   c->Runtime::destroy();
   if (c->hS) (void)hipHostFree(c->hS);
   if (c->hFlag) (void)hipHostFree(c->hFlag);
+  if (c->hFlagQ) (void)hipHostFree(c->hFlagQ);
   delete c;
   return WX_OK;
 }
@@ -347,7 +392,7 @@ int wx_timestep(wx_ctx *c, double *dt) {
   if (!c || !dt) return fail(WX_EARG, "null argument");
   int rc = ensure_stats(c);
   if (rc) return rc;
-  rc = launch_dtreduce(c->dP.ptr, c->dLeaves.ptr, c->dStat.ptr, c->dInstOff.ptr, 1, c->dS.ptr, 0, c->stream);
+  rc = launch_dtreduce(c->dP.ptr, c->tLeaves, c->dStat.ptr, c->tOff, 1, c->dS.ptr, 0, c->stream);
   if (rc) return fail(rc, "dtreduce launch failed");
   rc = sync_scalars(c);
   if (rc) return rc;
@@ -438,7 +483,7 @@ int wx_step(wx_ctx *c, int32_t nsteps, int32_t do_admesh_flag, double *time, dou
     if (rc) return rc;
     {
       ProfScope ps(c, KID_DTRED);      // timestep; time += dt
-      rc = launch_dtreduce(c->dP.ptr, c->dLeaves.ptr, c->dStat.ptr, c->dInstOff.ptr, 1, c->dS.ptr, 1, c->stream);
+      rc = launch_dtreduce(c->dP.ptr, c->tLeaves, c->dStat.ptr, c->tOff, 1, c->dS.ptr, 1, c->stream);
     }
     if (rc) return fail(rc, "dtreduce launch failed");
     z += (double)c->nleaf() * NX * NY;
