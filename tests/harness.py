@@ -57,11 +57,14 @@ def cell_scale(b: np.ndarray, gamma: float = 5. / 3.) -> np.ndarray:
     return sc
 
 
-def rel_err(a: np.ndarray, b: np.ndarray, mask=MASK):
-    """max over masked cells and variables of |a-b| / scale(cell, var), where
-    the scale is the cell's own magnitude of that variable (cell_scale)."""
+def rel_err(a: np.ndarray, b: np.ndarray, mask=MASK, nvar=None):
+    """max over masked cells and the first nvar variables (all if None) of
+    |a-b| / scale(cell, var), where the scale is the cell's own magnitude of
+    that variable (cell_scale)."""
     sc = cell_scale(b)
     d = np.abs(a - b) / np.maximum(sc, 1e-300)
+    if nvar is not None:
+        d = d[:, :nvar]
     d = d[:, :, mask]
     if d.size == 0:
         return 0.0, None

@@ -74,7 +74,14 @@ def test_adam_step_matches_reference():
     _, g = S.grad(m, x, t)
     S.train_steps(m, x, t, batch=512, steps=1, lr0=1e-3, lr1=1e-3)
     th1, _, _ = M.adam(th0.astype(np.float64), g.astype(np.float64), np.zeros(M.NPARAM), np.zeros(M.NPARAM), 1, 1e-3)
-    assert np.allclose(m.theta, th1, rtol=1e-5, atol=2e-7)
+    # Adam's first step moves each parameter by about lr * sign(g).  The
+    # training step recomputes the gradient, whose float atomics sum in a
+    # different order, so a gradient that is zero up to rounding may change
+    # sign: those parameters may differ by up to 2 lr; all others match.
+    clear = np.abs(g) > 1e-5 * np.abs(g).max()
+    assert clear.mean() > 0.99
+    assert np.allclose(m.theta[clear], th1[clear], rtol=1e-5, atol=2e-7)
+    assert np.all(np.abs(m.theta - th1) <= 2.0 * 1e-3 + 1e-6)
 
 
 def test_training_reduces_loss():
