@@ -32,7 +32,7 @@ struct wx_ctx : Mesh, Runtime {
   LeafRec *tLeaves = nullptr;
   int2 *tOpsII = nullptr, *tOpsI = nullptr;
   BlockRec *tBinfo = nullptr;
-  int32_t tLeafCap = 1;
+  int32_t tLeafCap = 0;
   DevBuf<double> dStat;
   DevBuf<int8_t> dFlag;
   DevBuf<MeshOp> dMeshOps;
@@ -57,22 +57,32 @@ struct wx_ctx : Mesh, Runtime {
 
 namespace {
 
+// The arena holds fixed regions for tLeafCap leaves (records, then the ops
+// of fills II and I) and for every block ID (block records, last); it is
+// reallocated only when the leaf count outgrows the regions.  One copy per
+// mesh change moves the header, the leaf regions (their unused tails
+// included) and the used block records.
 int upload_tables(wx_ctx *c) {
   const size_t nleaf = c->leaves.size();
-  const size_t bl = nleaf * sizeof(LeafRec), bo = nleaf * 8 * sizeof(int2), bb = c->binfo.size() * sizeof(BlockRec);
-  const size_t total = 2 * 256 + al256(bl) + 2 * al256(bo) + al256(bb);
   int rc;
-  if (total > c->dTab.n) {   // grow with room for the next mesh changes
-    if ((rc = c->dTab.ensure(total + total / 2))) return rc;
+  if (!c->tQ || nleaf > (size_t)c->tLeafCap) {
+    const size_t lcap = std::max<size_t>(1024, nleaf + nleaf / 4);
+    const size_t total = 512 + al256(lcap * sizeof(LeafRec)) + 2 * al256(lcap * 8 * sizeof(int2)) +
+                         al256(((size_t)c->cap + 1) * sizeof(BlockRec));
+    if ((rc = c->dTab.ensure(total))) return rc;
+    char *base = c->dTab.ptr;
+    c->tQ = (QueueHdr *)base;
+    c->tOff = (int32_t *)(base + 256);
+    c->tLeaves = (LeafRec *)(base + 512);
+    c->tOpsII = (int2 *)((char *)c->tLeaves + al256(lcap * sizeof(LeafRec)));
+    c->tOpsI = (int2 *)((char *)c->tOpsII + al256(lcap * 8 * sizeof(int2)));
+    c->tBinfo = (BlockRec *)((char *)c->tOpsI + al256(lcap * 8 * sizeof(int2)));
+    c->tLeafCap = (int32_t)lcap;
   }
   char *base = c->dTab.ptr;
-  c->tQ = (QueueHdr *)base;
-  c->tOff = (int32_t *)(base + 256);
-  c->tLeaves = (LeafRec *)(base + 512);
-  c->tOpsII = (int2 *)((char *)c->tLeaves + al256(bl));
-  c->tOpsI = (int2 *)((char *)c->tOpsII + al256(bo));
-  c->tBinfo = (BlockRec *)((char *)c->tOpsI + al256(bo));
-  c->tLeafCap = (int32_t)std::max<size_t>(nleaf, 1);
+  const size_t o2 = (char *)c->tOpsII - base, o1 = (char *)c->tOpsI - base, ob = (char *)c->tBinfo - base;
+  const size_t bl = nleaf * sizeof(LeafRec), bo = nleaf * 8 * sizeof(int2), bb = c->binfo.size() * sizeof(BlockRec);
+  const size_t total = ob + bb;
   if ((rc = stage_begin(c, total))) return rc;
   char *h = stage_take(c, total);
   if (!h) return fail(WX_EARG, "staging %zu bytes", total);
@@ -81,9 +91,9 @@ int upload_tables(wx_ctx *c) {
   memcpy(h, &qh, sizeof qh);
   memcpy(h + 256, off, sizeof off);
   memcpy(h + 512, c->leaves.data(), bl);
-  memcpy(h + 512 + al256(bl), c->opsII.data(), bo);
-  memcpy(h + 512 + al256(bl) + al256(bo), c->opsI.data(), bo);
-  memcpy(h + 512 + al256(bl) + 2 * al256(bo), c->binfo.data(), bb);
+  memcpy(h + o2, c->opsII.data(), bo);
+  memcpy(h + o1, c->opsI.data(), bo);
+  memcpy(h + ob, c->binfo.data(), bb);
   if ((rc = stage_push(c, base, h, total))) return rc;
   stage_end(c);
   return WX_OK;
@@ -248,17 +258,10 @@ int wx_create(const wx_params *p, wx_ctx **out) {
   for (int k = 0; k < 2 && !rc; ++k) rc = c->U[k].ensure(tiles);
   if (!rc) rc = c->dP.ensure(1);
   if (!rc) rc = c->dS.ensure(1);
-  if (!rc) rc = c->dTab.ensure(4096);   // an empty queue until the first tables
   if (!rc) rc = c->dStat.ensure((size_t)(c->cap + 1) * 2);
   if (!rc) rc = c->dFlag.ensure((size_t)c->cap + 1);
   if (rc) { wx_destroy(c); return rc; }
-  c->tQ = (QueueHdr *)c->dTab.ptr;
-  c->tOff = (int32_t *)(c->dTab.ptr + 256);
-  c->tLeaves = (LeafRec *)(c->dTab.ptr + 512);
-  c->tOpsII = c->tOpsI = (int2 *)(c->dTab.ptr + 1024);
-  c->tBinfo = (BlockRec *)(c->dTab.ptr + 2048);
   for (int k = 0; k < 2; ++k) (void)hipMemsetAsync(c->U[k].ptr, 0, tiles * sizeof(double), c->stream);
-  (void)hipMemsetAsync(c->dTab.ptr, 0, 4096, c->stream);
   (void)hipMemcpyAsync(c->dP.ptr, p, sizeof(wx_params), hipMemcpyHostToDevice, c->stream);
   DevScalars z{};
   (void)hipMemcpyAsync(c->dS.ptr, &z, sizeof z, hipMemcpyHostToDevice, c->stream);
