@@ -178,8 +178,9 @@ int wx_read_bin(wx_ctx *ctx, int32_t nfiles, const char *const *pointers_paths, 
  * parameters (p[0..n-1], same neq), tree, time and timestep, advanced
  * together.  The reference runs one mesh per process (the time loop
  * src/main.f90:69-106); an ensemble replaces n such runs.  Each stage is one
- * persistent launch over the compacted queue of every instance's leaf
- * blocks; the step can be captured in a hipGraph; host work per instance
+ * launch over the compacted queue of every instance's leaf blocks (one
+ * workgroup per block, or a persistent grid that the autotuner may choose);
+ * the step can be captured in a hipGraph; host work per instance
  * (list compilation, admesh) runs on worker threads (WX_HOST_THREADS, else
  * OMP_NUM_THREADS, at most 32).  Results of instance i are bit-identical to
  * running it alone in a wx_ctx with the same parameters and initial data.

@@ -8,13 +8,15 @@
 // leaf records and ghost ops in the region starting at leafbase[i].  After a
 // mesh change only the regions of instances whose mesh changed are
 // uploaded, and k_build_queue gathers all regions into the compacted queue.
-// Each stage is ONE persistent launch over that queue.  The step's
-// launches (dt reduction, the three stages, the flag gather) can be
-// captured in two hipGraphs (one per tile-buffer parity); as the kernels
-// read the queue length from device memory and the buffers are allocated at
-// full capacity up front, the graphs never need re-capturing when meshes
-// change.  Per-instance host work (table compilation, admesh) runs on a
-// pool of worker threads.
+// Each stage is ONE launch over that queue (one workgroup per entry, or a
+// persistent grid looping over it).  The step's launches (dt reduction, the
+// three stages, the flag gather) can be captured in two hipGraphs (one per
+// tile-buffer parity); as the kernels read the queue length from device
+// memory and the buffers are allocated at full capacity up front, the
+// graphs are re-captured only when the queue outgrows the span they cover.
+// Per-instance host work (table compilation, admesh) runs on a pool of
+// worker threads; wx_ens_create_groups splits the instances into groups
+// (each of the above per group) stepped concurrently from host threads.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
