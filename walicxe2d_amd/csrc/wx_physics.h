@@ -219,9 +219,16 @@ __device__ __forceinline__ int riemann(const wx_params *__restrict__ p, const do
 // Cooling.  src/coolingh.f90:41-76 rates, :88-165 ALOSS, :171-259 atomic.
 __device__ __forceinline__ double rate_alpha(double T) { return 2.55e-13 * pow(1.e4 / T, 0.79); }
 __device__ __forceinline__ double rate_colf(double T) { return 5.83e-11 * sqrt(T) * exp(-(157828. / T)); }
+// Recombination cooling rate, used only by ALOSS.  a**(-0.33333) is taken
+// as exp(-0.33333 log a) with the log the formula needs anyway: a few ulps
+// from the power function, at a fifth of its cost.  (The loss enters only
+// the temperature relaxation, which does not amplify it; the rates alpha
+// and colf, which the stiff neutral-fraction update amplifies ~2000-fold,
+// keep the power and exponential functions.)
 __device__ __forceinline__ double rate_betah(double T) {
   double a = 157890. / T;
-  return 1.133e-24 / sqrt(a) * (-0.0713 + 0.5 * log(a) + 0.640 * pow(a, -0.33333));
+  double la = log(a);
+  return 1.133e-24 / sqrt(a) * (-0.0713 + 0.5 * la + 0.640 * exp(-0.33333 * la));
 }
 
 __device__ __forceinline__ double aloss(double x1, double den, double dh0, double te0) {
@@ -276,10 +283,11 @@ __device__ __forceinline__ double aloss(double x1, double den, double dh0, doubl
     LOW = EREC + 7.033 * (EOI + EOII);
   }
   if (TE > 44770.) {
-    // TE**0.5 as sqrt
+    // TE**0.5 as sqrt; TE**0.63 and (TE*1e-5)**1.63 as exp(y log x) (a few
+    // ulps, in the loss only, as for betah)
     double BETAF = 1.3 * 1.42E-27 * sqrt(TE);
     double HIICOOL = DE * DHP * BETAF;
-    HIGH = (1.0455E-18 / pow(TE, 0.63)) * (1. - exp(-pow(TE * 1.E-5, 1.63))) * DE * den + HIICOOL;
+    HIGH = (1.0455E-18 / exp(0.63 * log(TE))) * (1. - exp(-exp(1.63 * log(TE * 1.E-5)))) * DE * den + HIICOOL;
   }
   return ECOLL + EION + LOW * (1. - FR) + HIGH * FR;
 }
