@@ -105,6 +105,10 @@ int launch_meshops(int, const MeshOp *ops, int nops, double *, hipStream_t) {
   return 0;
 }
 int launch_transpose(int, int, const double *, double *, int, int, hipStream_t) { return 0; }
+int launch_h2d(const CopyDesc *d, int n, hipStream_t) {
+  for (int k = 0; k < n; ++k) memmove(d[k].dst, d[k].src, d[k].bytes);
+  return 0;
+}
 int launch_fillI_cool_sur(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const int2 *ops,
                           const BlockRec *binfo, const double *Uin, double *Uout, double *stat, int8_t *flag,
                           DevScalars *S, SurDev *, hipStream_t st) {

@@ -83,6 +83,14 @@ struct MeshOp {
   int32_t pad[2];
 };
 
+// One host-to-device copy of an upload: `bytes` from src (the device alias of
+// mapped pinned host memory) to dst (device memory).
+struct CopyDesc {
+  void *dst;
+  const void *src;
+  uint64_t bytes;
+};
+
 // launch wrappers (wx_kernels.hip).  P and S are arrays indexed by
 // LeafRec::inst (one element for a single mesh); `grid` is the number of
 // workgroups of the persistent stage kernels.
@@ -114,6 +122,9 @@ int launch_dtreduce(const wx_params *P, const LeafRec *rec, const double *stat, 
 int launch_gather_flags(QueueRef q, const int32_t *inst_off, int grid, const int8_t *flag, int8_t *flagq,
                         const DevScalars *S, DevScalars *Sout, int ninst, hipStream_t stream);
 int launch_meshops(int neq, const MeshOp *ops, int nops, double *U, hipStream_t stream);
+// Performs the n copies described in d (itself in mapped host memory), one
+// launch for all: a kernel reads the pinned staging directly.
+int launch_h2d(const CopyDesc *d, int n, hipStream_t stream);
 int launch_transpose(int neq, int cap, const double *src, double *dst, int to_fortran, int ld,
                      hipStream_t stream);
 int launch_calcprim(const wx_params *P, int neq, int cap, const double *U, double *prim, hipStream_t stream);

@@ -185,7 +185,7 @@ int ensure_queue(EnsGroup *e) {
   const QueueHdr qh{e->nleaf_total, e->n, {0, 0}};
   if ((rc = stage_copy(e, e->dQ.ptr, &qh, sizeof qh))) return rc;
   if ((rc = stage_copy(e, e->dInstOff.ptr, e->inst_off.data(), (size_t)(e->n + 1) * sizeof(int32_t)))) return rc;
-  stage_end(e);
+  if ((rc = stage_end(e))) return rc;
   rc = launch_build_queue(e->dQ.ptr, e->dInstOff.ptr, e->dLeafBase.ptr, e->dRegRec.ptr, e->dRegOpsII.ptr,
                           e->dRegOpsI.ptr, e->dRec.ptr, e->dOpsII.ptr, e->dOpsI.ptr,
                           std::max(1, std::min((8 * e->nleaf_total + 255) / 256, 4096)), e->stream);
@@ -221,7 +221,7 @@ int ensure_stats(EnsGroup *e) {
   if ((rc = e->dDirty.ensure(e->dirty.size()))) return rc;
   if ((rc = stage_begin(e, al256(e->dirty.size() * sizeof(LeafRec))))) return rc;
   if ((rc = stage_copy(e, e->dDirty.ptr, e->dirty.data(), e->dirty.size() * sizeof(LeafRec)))) return rc;
-  stage_end(e);
+  if ((rc = stage_end(e))) return rc;
   ProfScope ps(e, KID_STAT);
   rc = launch_blockstat(e->dP.ptr, e->neq, e->dDirty.ptr, (int)e->dirty.size(), e->cur_u(), e->dStat.ptr,
                         e->dFlag.ptr, e->stream);
@@ -323,7 +323,7 @@ int admesh_all(EnsGroup *e, int *changed) {
   if ((rc = e->dMeshOps.ensure(e->ops.size()))) return rc;
   if ((rc = stage_begin(e, al256(e->ops.size() * sizeof(MeshOp))))) return rc;
   if ((rc = stage_copy(e, e->dMeshOps.ptr, e->ops.data(), e->ops.size() * sizeof(MeshOp)))) return rc;
-  stage_end(e);
+  if ((rc = stage_end(e))) return rc;
   ProfScope ps(e, KID_MESH);
   rc = launch_meshops(e->neq, e->dMeshOps.ptr, (int)e->ops.size(), e->cur_u(), e->stream);
   if (rc) return fail(rc, "meshops launch failed");

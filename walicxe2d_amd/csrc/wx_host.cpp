@@ -59,57 +59,101 @@ namespace {
 
 // The arena holds fixed regions for tLeafCap leaves (records, then the ops
 // of fills II and I) and for every block ID (block records, last); it is
-// reallocated only when the leaf count outgrows the regions.  One copy per
-// mesh change moves the header, the leaf regions (their unused tails
-// included) and the used block records.
-int upload_tables(wx_ctx *c) {
+// reallocated only when the leaf count outgrows the regions.
+int arena_fit(wx_ctx *c) {
   const size_t nleaf = c->leaves.size();
-  int rc;
-  if (!c->tQ || nleaf > (size_t)c->tLeafCap) {
-    const size_t lcap = std::max<size_t>(1024, nleaf + nleaf / 4);
-    const size_t total = 512 + al256(lcap * sizeof(LeafRec)) + 2 * al256(lcap * 8 * sizeof(int2)) +
-                         al256(((size_t)c->cap + 1) * sizeof(BlockRec));
-    if ((rc = c->dTab.ensure(total))) return rc;
-    char *base = c->dTab.ptr;
-    c->tQ = (QueueHdr *)base;
-    c->tOff = (int32_t *)(base + 256);
-    c->tLeaves = (LeafRec *)(base + 512);
-    c->tOpsII = (int2 *)((char *)c->tLeaves + al256(lcap * sizeof(LeafRec)));
-    c->tOpsI = (int2 *)((char *)c->tOpsII + al256(lcap * 8 * sizeof(int2)));
-    c->tBinfo = (BlockRec *)((char *)c->tOpsI + al256(lcap * 8 * sizeof(int2)));
-    c->tLeafCap = (int32_t)lcap;
-  }
+  if (c->tQ && nleaf <= (size_t)c->tLeafCap) return WX_OK;
+  const size_t lcap = std::max<size_t>(1024, nleaf + nleaf / 4);
+  const size_t total = 512 + al256(lcap * sizeof(LeafRec)) + 2 * al256(lcap * 8 * sizeof(int2)) +
+                       al256(((size_t)c->cap + 1) * sizeof(BlockRec));
+  int rc = c->dTab.ensure(total);
+  if (rc) return rc;
   char *base = c->dTab.ptr;
-  const size_t o2 = (char *)c->tOpsII - base, o1 = (char *)c->tOpsI - base, ob = (char *)c->tBinfo - base;
-  const size_t bl = nleaf * sizeof(LeafRec), bo = nleaf * 8 * sizeof(int2), bb = c->binfo.size() * sizeof(BlockRec);
-  const size_t total = ob + bb;
-  if ((rc = stage_begin(c, total))) return rc;
-  char *h = stage_take(c, total);
-  if (!h) return fail(WX_EARG, "staging %zu bytes", total);
+  c->tQ = (QueueHdr *)base;
+  c->tOff = (int32_t *)(base + 256);
+  c->tLeaves = (LeafRec *)(base + 512);
+  c->tOpsII = (int2 *)((char *)c->tLeaves + al256(lcap * sizeof(LeafRec)));
+  c->tOpsI = (int2 *)((char *)c->tOpsII + al256(lcap * 8 * sizeof(int2)));
+  c->tBinfo = (BlockRec *)((char *)c->tOpsI + al256(lcap * 8 * sizeof(int2)));
+  c->tLeafCap = (int32_t)lcap;
+  return WX_OK;
+}
+
+// First half of a table upload: the queue header, the leaf records and the
+// block records (what the block statistics, the dt reduction and stage A
+// read).
+int upload_leaves(wx_ctx *c) {
+  int rc = arena_fit(c);
+  if (rc) return rc;
+  const size_t nleaf = c->leaves.size();
+  const size_t bl = nleaf * sizeof(LeafRec), bb = c->binfo.size() * sizeof(BlockRec);
+  if ((rc = stage_begin(c, al256(512 + bl) + al256(bb)))) return rc;
+  char *h = stage_take(c, 512 + bl);
+  char *hb = stage_take(c, bb);
+  if (!h || !hb) return fail(WX_EARG, "staging %zu bytes", 512 + bl + bb);
   const QueueHdr qh{(int32_t)nleaf, 1, {0, 0}};
   const int32_t off[2] = {0, (int32_t)nleaf};
+  memset(h, 0, 512);
   memcpy(h, &qh, sizeof qh);
   memcpy(h + 256, off, sizeof off);
   memcpy(h + 512, c->leaves.data(), bl);
-  memcpy(h + o2, c->opsII.data(), bo);
-  memcpy(h + o1, c->opsI.data(), bo);
-  memcpy(h + ob, c->binfo.data(), bb);
-  if ((rc = stage_push(c, base, h, total))) return rc;
-  stage_end(c);
+  memcpy(hb, c->binfo.data(), bb);
+  if ((rc = stage_push(c, c->dTab.ptr, h, 512 + bl))) return rc;
+  if ((rc = stage_push(c, c->tBinfo, hb, bb))) return rc;
+  return stage_end(c);
+}
+
+// Second half: the ops of both fills, and the leaf records again when the
+// list set imposition flags in them.
+int upload_ops(wx_ctx *c) {
+  const size_t nleaf = c->leaves.size();
+  const size_t bo = nleaf * 8 * sizeof(int2), bl = nleaf * sizeof(LeafRec);
+  const bool flags = c->p.otherb != 0;
+  int rc = stage_begin(c, 2 * al256(bo) + (flags ? al256(bl) : 0));
+  if (rc) return rc;
+  char *h2 = stage_take(c, bo), *h1 = stage_take(c, bo);
+  char *hl = flags ? stage_take(c, bl) : nullptr;
+  if (!h2 || !h1 || (flags && !hl)) return fail(WX_EARG, "staging %zu bytes", 2 * bo + bl);
+  memcpy(h2, c->opsII.data(), bo);
+  memcpy(h1, c->opsI.data(), bo);
+  if ((rc = stage_push(c, c->tOpsII, h2, bo))) return rc;
+  if ((rc = stage_push(c, c->tOpsI, h1, bo))) return rc;
+  if (flags) {
+    memcpy(hl, c->leaves.data(), bl);
+    if ((rc = stage_push(c, c->tLeaves, hl, bl))) return rc;
+  }
+  return stage_end(c);
+}
+
+// After a mesh change the tables are rebuilt in two halves, so that a step
+// can launch the statistics, the dt reduction and stage A (which need only
+// the leaf records) before the ghost list is located and compiled.
+int tables_leaves(wx_ctx *c) {
+  HostTimer ht(c, HID_TABLES);
+  int rc = compile_leaves(c);
+  return rc ? rc : upload_leaves(c);
+}
+
+int tables_ops(wx_ctx *c) {
+  HostTimer ht(c, HID_TABLES);
+  if (!c->bounds_external) locatebounds(c);
+  int rc = compile_ops(c);
+  if (rc) return rc;
+  if ((rc = upload_ops(c))) return rc;
+  c->tables_valid = true;
   return WX_OK;
 }
 
 int ensure_tables(wx_ctx *c) {
   if (c->tables_valid) return WX_OK;
-  HostTimer ht(c, HID_TABLES);
-  int rc = refresh_tables(c);
-  if (rc) return rc;
-  return upload_tables(c);
+  int rc = tables_leaves(c);
+  return rc ? rc : tables_ops(c);
 }
 
-int ensure_stats(wx_ctx *c) {
-  int rc = ensure_tables(c);
-  if (rc) return rc;
+// Block statistics of the leaves that lack them (all, or the new leaves of
+// the last mesh change); the leaf records must be current.
+int update_stats(wx_ctx *c) {
+  int rc;
   if (c->stats_all_stale) {
     ProfScope ps(c, KID_STAT);
     rc = launch_blockstat(c->dP.ptr, c->neq, c->tLeaves, c->nleaf(), c->cur_u(), c->dStat.ptr,
@@ -128,12 +172,17 @@ int ensure_stats(wx_ctx *c) {
   if ((rc = c->dDirty.ensure(c->dirty.size()))) return rc;
   if ((rc = stage_begin(c, al256(c->dirty.size() * sizeof(LeafRec))))) return rc;
   if ((rc = stage_copy(c, c->dDirty.ptr, c->dirty.data(), c->dirty.size() * sizeof(LeafRec)))) return rc;
-  stage_end(c);
+  if ((rc = stage_end(c))) return rc;
   ProfScope ps(c, KID_STAT);
   rc = launch_blockstat(c->dP.ptr, c->neq, c->dDirty.ptr, (int)c->dirty.size(), c->cur_u(), c->dStat.ptr,
                         c->dFlag.ptr, c->stream);
   if (rc) return fail(rc, "blockstat launch failed");
   return WX_OK;
+}
+
+int ensure_stats(wx_ctx *c) {
+  int rc = ensure_tables(c);
+  return rc ? rc : update_stats(c);
 }
 
 int sync_scalars(wx_ctx *c) {
@@ -146,14 +195,19 @@ int sync_scalars(wx_ctx *c) {
   return WX_OK;
 }
 
-int run_stages(wx_ctx *c) {
+// tstep: stage A, then stage B and the fill (which need the ops tables).
+int run_stage_a(wx_ctx *c) {
+  ProfScope ps(c, KID_STAGEA);
+  int rc = launch_stageA(c->dP.ptr, c->neq, c->queue(), c->nleaf(), 256, 0, c->cur_u(), c->other_u(), c->dS.ptr,
+                         c->stream);
+  if (rc) return fail(rc, "stage launch failed: %s", hipGetErrorString(hipGetLastError()));
+  return WX_OK;
+}
+
+int run_stages_bf(wx_ctx *c) {
   const int g = c->nleaf();   // one workgroup per queue entry
   int rc;
   {
-    ProfScope ps(c, KID_STAGEA);
-    rc = launch_stageA(c->dP.ptr, c->neq, c->queue(), g, 256, 0, c->cur_u(), c->other_u(), c->dS.ptr, c->stream);
-  }
-  if (!rc) {
     ProfScope ps(c, KID_STAGEB);
     rc = launch_stageB(c->dP.ptr, c->neq, c->queue(), g, 256, 0, c->tOpsII, c->tBinfo, c->cur_u(), c->other_u(),
                        c->dS.ptr, c->stream);
@@ -174,13 +228,18 @@ int run_stages(wx_ctx *c) {
   return WX_OK;
 }
 
+int run_stages(wx_ctx *c) {
+  int rc = run_stage_a(c);
+  return rc ? rc : run_stages_bf(c);
+}
+
 int run_meshops(wx_ctx *c, const std::vector<MeshOp> &ops) {
   if (ops.empty()) return WX_OK;
   int rc = c->dMeshOps.ensure(ops.size());
   if (rc) return rc;
   if ((rc = stage_begin(c, al256(ops.size() * sizeof(MeshOp))))) return rc;
   if ((rc = stage_copy(c, c->dMeshOps.ptr, ops.data(), ops.size() * sizeof(MeshOp)))) return rc;
-  stage_end(c);
+  if ((rc = stage_end(c))) return rc;
   {
     ProfScope ps(c, KID_MESH);
     rc = launch_meshops(c->neq, c->dMeshOps.ptr, (int)ops.size(), c->cur_u(), c->stream);
@@ -482,7 +541,11 @@ int wx_step(wx_ctx *c, int32_t nsteps, int32_t do_admesh_flag, double *time, dou
   if (!c || nsteps < 0) return fail(WX_EARG, "bad argument");
   double z = 0.0;
   for (int s = 0; s < nsteps; ++s) {
-    int rc = ensure_stats(c);          // rebuilds tables after a mesh change
+    // after a mesh change the ghost list is located and compiled while the
+    // device runs the statistics, the dt reduction and stage A
+    const bool split = !c->tables_valid;
+    int rc = split ? tables_leaves(c) : WX_OK;
+    if (!rc) rc = update_stats(c);
     if (rc) return rc;
     {
       ProfScope ps(c, KID_DTRED);      // timestep; time += dt
@@ -490,7 +553,9 @@ int wx_step(wx_ctx *c, int32_t nsteps, int32_t do_admesh_flag, double *time, dou
     }
     if (rc) return fail(rc, "dtreduce launch failed");
     z += (double)c->nleaf() * NX * NY;
-    rc = run_stages(c);                // tstep
+    rc = run_stage_a(c);               // tstep
+    if (!rc && split) rc = tables_ops(c);
+    if (!rc) rc = run_stages_bf(c);
     if (rc) return rc;
     if (do_admesh_flag && c->p.nlevs > 1) {
       rc = do_admesh(c, nullptr);      // admesh

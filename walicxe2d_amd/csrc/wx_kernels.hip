@@ -976,6 +976,35 @@ int launch_meshops(int neq, const MeshOp *ops, int nops, double *U, hipStream_t 
   return hip_status();
 }
 
+// Upload copies: workgroups (x, y) copy part x of copy y, in 16-byte words
+// when both ends and the size allow, else 4-byte words, else bytes.
+__global__ __launch_bounds__(256) void k_h2d(const CopyDesc *__restrict__ d, int base) {
+  const CopyDesc c = d[base + blockIdx.y];
+  const uint64_t al = (uint64_t)c.dst | (uint64_t)c.src | c.bytes;
+  const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x, st = (size_t)gridDim.x * blockDim.x;
+  if ((al & 15) == 0) {
+    const uint4 *s = (const uint4 *)c.src;
+    uint4 *o = (uint4 *)c.dst;
+    for (size_t i = t0; i < c.bytes / 16; i += st) o[i] = s[i];
+  } else if ((al & 3) == 0) {
+    const uint32_t *s = (const uint32_t *)c.src;
+    uint32_t *o = (uint32_t *)c.dst;
+    for (size_t i = t0; i < c.bytes / 4; i += st) o[i] = s[i];
+  } else {
+    const uint8_t *s = (const uint8_t *)c.src;
+    uint8_t *o = (uint8_t *)c.dst;
+    for (size_t i = t0; i < c.bytes; i += st) o[i] = s[i];
+  }
+}
+
+int launch_h2d(const CopyDesc *d, int n, hipStream_t stream) {
+  for (int base = 0; base < n; base += 65535) {
+    const int m = n - base < 65535 ? n - base : 65535;
+    hipLaunchKernelGGL(k_h2d, dim3(16, m), dim3(256), 0, stream, d, base);
+  }
+  return hip_status();
+}
+
 int launch_transpose(int neq, int cap, const double *src, double *dst, int to_fortran, int ld, hipStream_t stream) {
   WX_DISPATCH(neq, hipLaunchKernelGGL(k_transpose<N>, dim3(1024), dim3(256), 0, stream, cap, src, dst, to_fortran, ld));
   return hip_status();

@@ -89,12 +89,9 @@ void locatebounds(Mesh *c) {
 }
 
 // ---------------------------------------------------------------------
-// Compile the ordered list into per-segment gather ops (last writer wins,
-// jet-imposition order folded into two flags).  Ops are written in place:
-// op.x = source block, op.y = kind | offset << GF_OFF_SHIFT, and the list
-// position of the writer is kept beside each op for the imposition flags.
-// All scratch lives in the Mesh and is reused from call to call.
-int compile_tables(Mesh *c) {
+// The leaf records (block-ID order, imposition flags cleared), the leaf
+// position of every block ID and the block records, from the tree.
+int compile_leaves(Mesh *c) {
   const int cap = c->nbmax;   // IDs above nbmax hold no block
   c->leaves.clear();
   c->lidx.assign((size_t)cap + 1, -1);
@@ -113,6 +110,17 @@ int compile_tables(Mesh *c) {
     if (r.level < 1 || r.level > c->p.nlevs) return fail(WX_EARG, "block %d has level %d", nb, r.level);
     c->leaves.push_back(r);
   }
+  return WX_OK;
+}
+
+// Compile the ordered list into per-segment gather ops (last writer wins,
+// jet-imposition order folded into two flags).  Ops are written in place:
+// op.x = source block, op.y = kind | offset << GF_OFF_SHIFT, and the list
+// position of the writer is kept beside each op for the imposition flags.
+// Needs compile_leaves' records of the same tree; sets their imposition
+// flags.  All scratch lives in the Mesh and is reused from call to call.
+int compile_ops(Mesh *c) {
+  const int cap = c->nbmax;
   const int nleaf = (int)c->leaves.size();
   const size_t nseg = (size_t)nleaf * 8;
   c->opsII.assign(nseg, int2{0, GK_NONE});
@@ -350,6 +358,11 @@ bool params_ok(const wx_params *p, std::string &why) {
   for (int k = 0; k < 4; ++k)
     if (p->bc[k] < 0 || p->bc[k] > 1) { why = "bc must be 0 (outflow) or 1 (reflective)"; return false; }
   return true;
+}
+
+int compile_tables(Mesh *c) {
+  int rc = compile_leaves(c);
+  return rc ? rc : compile_ops(c);
 }
 
 int refresh_tables(Mesh *c) {

@@ -19,6 +19,7 @@ void Runtime::destroy() {
   for (auto &st : stage) {
     if (st.ev) (void)hipEventDestroy(st.ev);
     if (st.buf) (void)hipHostFree(st.buf);
+    if (st.hdesc) (void)hipHostFree(st.hdesc);
     st = Stage{};
   }
   prof_collect(this);
@@ -28,10 +29,25 @@ void Runtime::destroy() {
   stream = nullptr;
 }
 
+// Mapped, coherent pinned memory and its device alias.
+static int pinned_alloc(size_t bytes, char **host, char **dev) {
+  *host = *dev = nullptr;
+  if (hipHostMalloc((void **)host, bytes, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+    *host = nullptr;
+    return fail(WX_ENOMEM, "pinned staging %zu bytes", bytes);
+  }
+  if (hipHostGetDevicePointer((void **)dev, *host, 0) != hipSuccess) {
+    (void)hipHostFree(*host);
+    *host = nullptr;
+    return fail(WX_EHIP, "device alias of the pinned staging");
+  }
+  return WX_OK;
+}
+
 // Opens the next staging buffer for `total` bytes (waits only if that
 // buffer's previous copies have not finished).
 int stage_begin(Runtime *r, size_t total) {
-  r->stage_cur ^= 1;
+  r->stage_cur = (r->stage_cur + 1) % Runtime::NSTAGE;
   Runtime::Stage &st = r->stage[r->stage_cur];
   if (st.pending) {
     (void)hipEventSynchronize(st.ev);
@@ -41,13 +57,15 @@ int stage_begin(Runtime *r, size_t total) {
     return fail(WX_EHIP, "hipEventCreate failed");
   if (st.cap < total) {
     if (st.buf) (void)hipHostFree(st.buf);
-    st.buf = nullptr;
+    st.buf = st.dbuf = nullptr;
     st.cap = 0;
-    size_t want = total + total / 2 + 4096;
-    if (hipHostMalloc((void **)&st.buf, want) != hipSuccess) return fail(WX_ENOMEM, "pinned staging %zu bytes", want);
+    const size_t want = total + total / 2 + 4096;
+    int rc = pinned_alloc(want, &st.buf, &st.dbuf);
+    if (rc) return rc;
     st.cap = want;
   }
   st.used = 0;
+  st.desc.clear();
   return WX_OK;
 }
 
@@ -57,8 +75,8 @@ int stage_copy(Runtime *r, void *dst, const void *src, size_t bytes) {
   if (st.used + bytes > st.cap) return fail(WX_EARG, "staging overflow (%zu + %zu > %zu)", st.used, bytes, st.cap);
   char *p = st.buf + st.used;
   memcpy(p, src, bytes);
+  st.desc.push_back(CopyDesc{dst, st.dbuf + st.used, (uint64_t)bytes});
   st.used += al256(bytes);
-  HIPCHK(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, r->stream));
   return WX_OK;
 }
 
@@ -72,14 +90,35 @@ char *stage_take(Runtime *r, size_t bytes) {
 
 int stage_push(Runtime *r, void *dst, const char *host, size_t bytes) {
   if (!bytes) return WX_OK;
-  HIPCHK(hipMemcpyAsync(dst, host, bytes, hipMemcpyHostToDevice, r->stream));
+  Runtime::Stage &st = r->stage[r->stage_cur];
+  if (host < st.buf || host + bytes > st.buf + st.cap) return fail(WX_EARG, "stage_push outside the open staging buffer");
+  st.desc.push_back(CopyDesc{dst, st.dbuf + (host - st.buf), (uint64_t)bytes});
   return WX_OK;
 }
 
-void stage_end(Runtime *r) {
+int stage_end(Runtime *r) {
   Runtime::Stage &st = r->stage[r->stage_cur];
-  (void)hipEventRecord(st.ev, r->stream);
+  const size_t n = st.desc.size();
+  if (n) {
+    if (st.dcap < n) {   // the buffer's previous round has finished (stage_begin)
+      if (st.hdesc) (void)hipHostFree(st.hdesc);
+      st.hdesc = st.ddesc = nullptr;
+      st.dcap = 0;
+      const size_t want = n + n / 2 + 16;
+      char *h = nullptr, *d = nullptr;
+      int rc = pinned_alloc(want * sizeof(CopyDesc), &h, &d);
+      if (rc) return rc;
+      st.hdesc = (CopyDesc *)h;
+      st.ddesc = (CopyDesc *)d;
+      st.dcap = want;
+    }
+    memcpy(st.hdesc, st.desc.data(), n * sizeof(CopyDesc));
+    int rc = launch_h2d(st.ddesc, (int)n, r->stream);
+    if (rc) return fail(rc, "upload launch failed: %s", hipGetErrorString(hipGetLastError()));
+  }
+  HIPCHK(hipEventRecord(st.ev, r->stream));
   st.pending = true;
+  return WX_OK;
 }
 
 static hipEvent_t take_event(Runtime *r) {

@@ -90,7 +90,11 @@ struct Mesh {
 bool params_ok(const wx_params *p, std::string &why);
 // src/locatebounds.f90:29-540 (MPIP forms, one rank) from the tree.
 void locatebounds(Mesh *m);
-// The ordered list -> per-leaf gather ops (last writer of every segment).
+// The ordered list -> per-leaf gather ops (last writer of every segment):
+// compile_leaves (leaf and block records from the tree) then compile_ops
+// (the ops from the list, and the leaves' imposition flags).
+int compile_leaves(Mesh *m);
+int compile_ops(Mesh *m);
 int compile_tables(Mesh *m);
 // locatebounds (unless the list is external) + compile_tables when stale.
 int refresh_tables(Mesh *m);
@@ -112,10 +116,21 @@ enum { KID_STAGEA = 0, KID_STAGEB = 1, KID_FILLI = 2, KID_DTRED = 3, KID_STAT = 
 
 struct Runtime {
   hipStream_t stream = nullptr;
-  // pinned host staging for uploads: two buffers, each guarded by an event,
-  // so a table upload never waits for the device
-  struct Stage { char *buf = nullptr; size_t cap = 0, used = 0; hipEvent_t ev = nullptr; bool pending = false; };
-  Stage stage[2];
+  // Upload staging: a ring of four mapped pinned buffers, each guarded by an
+  // event.  The copies of one staging round are recorded as descriptors and
+  // performed by one kernel launch (k_h2d reads the pinned memory directly)
+  // at stage_end.
+  struct Stage {
+    char *buf = nullptr, *dbuf = nullptr;   // host pointer, device alias
+    size_t cap = 0, used = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+    std::vector<CopyDesc> desc;
+    CopyDesc *hdesc = nullptr, *ddesc = nullptr;
+    size_t dcap = 0;
+  };
+  static constexpr int NSTAGE = 4;
+  Stage stage[NSTAGE];
   int stage_cur = 0;
   // per-kernel timing with HIP events on this stream
   bool prof = false;
@@ -135,7 +150,8 @@ int stage_copy(Runtime *r, void *dst, const void *src, size_t bytes);
 // (nullptr if it does not fit); stage_push enqueues its copy to dst.
 char *stage_take(Runtime *r, size_t bytes);
 int stage_push(Runtime *r, void *dst, const char *host, size_t bytes);
-void stage_end(Runtime *r);
+// Launches the round's copies and records the buffer's event.
+int stage_end(Runtime *r);
 void prof_collect(Runtime *r);
 int prof_report(Runtime *r, int32_t enable, double *ms, int64_t *counts, int32_t n);
 
