@@ -219,14 +219,15 @@ __device__ __forceinline__ int riemann(const wx_params *__restrict__ p, const do
 // Cooling.  src/coolingh.f90:41-76 rates, :88-165 ALOSS, :171-259 atomic.
 __device__ __forceinline__ double rate_alpha(double T) { return 2.55e-13 * pow(1.e4 / T, 0.79); }
 __device__ __forceinline__ double rate_colf(double T) { return 5.83e-11 * sqrt(T) * exp(-(157828. / T)); }
-// Recombination cooling rate, used only by ALOSS.  a**(-0.33333) is taken
-// as exp(-0.33333 log a) with the log the formula needs anyway: a few ulps
-// from the power function, at a fifth of its cost.  (The loss enters only
-// the temperature relaxation, which does not amplify it; the rates alpha
-// and colf, which the stiff neutral-fraction update amplifies ~2000-fold,
-// keep the power and exponential functions.)
-__device__ __forceinline__ double rate_betah(double T) {
-  double a = 157890. / T;
+// Recombination cooling rate, used only by ALOSS (rT = 1/T).
+// a**(-0.33333) is taken as exp(-0.33333 log a) with the log the formula
+// needs anyway: a few ulps from the power function, at a fifth of its cost.
+// (The loss enters only the temperature relaxation, which does not amplify
+// it; the rates alpha and colf, which the stiff neutral-fraction update
+// amplifies ~2000-fold, keep the power and exponential functions and their
+// divisions.)
+__device__ __forceinline__ double rate_betah(double T, double rT) {
+  double a = 157890. * rT;
   double la = log(a);
   return 1.133e-24 / sqrt(a) * (-0.0713 + 0.5 * la + 0.640 * exp(-0.33333 * la));
 }
@@ -237,6 +238,8 @@ __device__ __forceinline__ double aloss(double x1, double den, double dh0, doubl
   const double D0 = 0.5856, D1 = 1.55083e-5, D2 = -9.669e-12, D3 = 5.716e-19;
   const double ENK = 118409., EN = 1.634E-11;
   double TE = dmax(te0, 1000.);
+  // x / TE as x * (1 / TE) in the loss terms (within an ulp)
+  const double TM = 1. / TE;
   double DH = den;
   double DHP = (1. - x1) * DH;
   double DE = DHP + 1.E-4 * DH;
@@ -251,10 +254,10 @@ __device__ __forceinline__ double aloss(double x1, double den, double dh0, doubl
     double FRAC = (TE - 55000.) / 17000.;
     OMEGA = (1. - FRAC) * OMEGAL + FRAC * OMEGAH;
   }
-  double QLA = 8.6287E-6 / (2. * sqrt(TE)) * OMEGA * exp(-(ENK / TE));
+  double QLA = 8.6287E-6 / (2. * sqrt(TE)) * OMEGA * exp(-(ENK * TM));
   double ECOLL = DE * dh0 * QLA * EN;
   ECOLL = dmax(ECOLL, 0.);
-  double CION = 5.834E-11 * sqrt(TE) * exp(-(1.579E5 / TE));
+  double CION = 5.834E-11 * sqrt(TE) * exp(-(1.579E5 * TM));
   double EION = DE * dh0 * CION * XION;
   double FR = 0.0;
   if (TE <= 44770.) FR = 0.;
@@ -270,9 +273,8 @@ __device__ __forceinline__ double aloss(double x1, double den, double dh0, doubl
   // as the routine's product with a zero weight).
   double LOW = 0.0, HIGH = 0.0;
   if (TE < 54770.) {
-    double EREC = DE * DHP * (rate_betah(TE));
+    double EREC = DE * DHP * (rate_betah(TE, TM));
     EREC = dmax(EREC, 0.);
-    double TM = 1. / TE;
     double T2 = TM * TM;
     // 10.**x of the routine as exp10 (within an ulp of the power function,
     // at a fraction of its cost)
