@@ -74,10 +74,14 @@ int launch_stageB(const wx_params *, int, QueueRef q, int grid, int bs, int pers
   return 0;
 }
 int launch_fillI_cool(const wx_params *P, int, QueueRef q, int grid, int bs, int persist, const int2 *,
-                      const BlockRec *, const double *, double *, double *stat, int8_t *flag, DevScalars *,
-                      hipStream_t) {
+                      const BlockRec *, const double *, double *, double *stat, int8_t *flag, DevScalars *S,
+                      FlagOut fo, hipStream_t) {
   check_shape(q, grid, bs, persist);
   write_flags(P, q.rec, q.hdr->nleaf, (signed char *)flag, stat);
+  if (fo.flagq) {
+    for (int l = 0; l < q.hdr->nleaf; ++l) fo.flagq[fo.inst_off[q.rec[l].inst] + q.rec[l].lidx] = flag[q.rec[l].id];
+    for (int i = 0; i < fo.ninst; ++i) fo.Sout[i] = S[i];
+  }
   return 0;
 }
 int launch_blockstat(const wx_params *P, int, const LeafRec *leaves, int nleaf, const double *, double *stat,
@@ -111,8 +115,8 @@ int launch_h2d(const CopyDesc *d, int n, hipStream_t) {
 }
 int launch_fillI_cool_sur(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const int2 *ops,
                           const BlockRec *binfo, const double *Uin, double *Uout, double *stat, int8_t *flag,
-                          DevScalars *S, SurDev *, hipStream_t st) {
-  return launch_fillI_cool(P, neq, q, grid, bs, persist, ops, binfo, Uin, Uout, stat, flag, S, st);
+                          DevScalars *S, SurDev *, FlagOut fo, hipStream_t st) {
+  return launch_fillI_cool(P, neq, q, grid, bs, persist, ops, binfo, Uin, Uout, stat, flag, S, fo, st);
 }
 int sur_upload(SurDev *, const wx_sur_model *, const wx_params *, hipStream_t) { return 0; }
 int sur_counters(SurDev *, hipStream_t, int, int64_t *c, int64_t *f) {

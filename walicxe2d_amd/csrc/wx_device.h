@@ -83,6 +83,16 @@ struct MeshOp {
   int32_t pad[2];
 };
 
+// Where the fill's epilogue also puts the refinement flags and the scalars
+// for the host (both in mapped pinned memory): flagq[inst_off[rec.inst] +
+// rec.lidx] = the entry's flag, Sout[0 .. ninst) = S.  flagq == nullptr: off.
+struct FlagOut {
+  const int32_t *inst_off;
+  int8_t *flagq;
+  DevScalars *Sout;
+  int32_t ninst;
+};
+
 // One host-to-device copy of an upload: `bytes` from src (the device alias of
 // mapped pinned host memory) to dst (device memory).
 struct CopyDesc {
@@ -109,7 +119,7 @@ int launch_stageB(const wx_params *P, int neq, QueueRef q, int grid, int bs, int
                   const BlockRec *binfo, double *U, const double *Q, DevScalars *S, hipStream_t stream);
 int launch_fillI_cool(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const int2 *ops,
                       const BlockRec *binfo, const double *Uin, double *Uout, double *stat, int8_t *flag,
-                      DevScalars *S, hipStream_t stream);
+                      DevScalars *S, FlagOut fo, hipStream_t stream);
 // Statistics of an explicit list of leaf records (one workgroup each).
 int launch_blockstat(const wx_params *P, int neq, const LeafRec *leaves, int nleaf, const double *U,
                      double *stat, int8_t *flag, hipStream_t stream);

@@ -54,7 +54,7 @@ struct EnsGroup : Runtime {
   DevBuf<int8_t> dFlag;
   DevBuf<MeshOp> dMeshOps;
   DevBuf<LeafRec> dDirty;
-  // mapped pinned mirrors written by k_gather_flags
+  // mapped pinned mirrors written by the fill's epilogue (or k_gather_flags)
   DevScalars *hS = nullptr, *hS_dev = nullptr;
   int8_t *hFlagQ = nullptr, *hFlagQ_dev = nullptr;
   // queue bookkeeping
@@ -250,20 +250,15 @@ int launch_step(EnsGroup *e, int par, bool for_graph) {
     rc = launch_stageB(e->dP.ptr, e->neq, q, sh[1].grid, e->bs[1], sh[1].persist, e->dOpsII.ptr, e->dBinfo.ptr, Ucur, Uoth,
                        e->dS.ptr, e->stream);
   }
-  if (!rc) {
+  if (!rc) {   // its epilogue also writes the flags (queue order) and the scalars for the host
     ProfScope ps(e, prof ? KID_FILLI : -1);
+    const FlagOut fo{e->dInstOff.ptr, e->hFlagQ_dev, e->hS_dev, e->n};
     if (e->use_sur)
       rc = launch_fillI_cool_sur(e->dP.ptr, e->neq, q, sh[2].grid, e->bs[2], sh[2].persist, e->dOpsI.ptr, e->dBinfo.ptr, Ucur,
-                                 Uoth, e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, e->dSur.ptr, e->stream);
+                                 Uoth, e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, e->dSur.ptr, fo, e->stream);
     else
       rc = launch_fillI_cool(e->dP.ptr, e->neq, q, sh[2].grid, e->bs[2], sh[2].persist, e->dOpsI.ptr, e->dBinfo.ptr, Ucur,
-                             Uoth, e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, e->stream);
-  }
-  if (!rc) {
-    const int g = for_graph ? std::max(1, (int)std::min<size_t>((e->nrec + 255) / 256, 1024))
-                            : std::max(1, std::min((e->nleaf_total + 255) / 256, 1024));
-    rc = launch_gather_flags(q, e->dInstOff.ptr, g, e->dFlag.ptr, e->hFlagQ_dev, e->dS.ptr, e->hS_dev, e->n,
-                             e->stream);
+                             Uoth, e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, fo, e->stream);
   }
   if (rc) return fail(rc, "step launch failed: %s", hipGetErrorString(hipGetLastError()));
   return WX_OK;
@@ -570,10 +565,11 @@ int grp_tune(EnsGroup *e, int32_t reps, int32_t *grids, int32_t *threads, double
                                e->dS.ptr, e->stream);
           else if (e->use_sur)
             rc = launch_fillI_cool_sur(e->dP.ptr, e->neq, q, g, bsz, pers, e->dOpsI.ptr, e->dBinfo.ptr, e->cur_u(),
-                                       scratch.ptr, e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, e->dSur.ptr, e->stream);
+                                       scratch.ptr, e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, e->dSur.ptr, FlagOut{},
+                                       e->stream);
           else
             rc = launch_fillI_cool(e->dP.ptr, e->neq, q, g, bsz, pers, e->dOpsI.ptr, e->dBinfo.ptr, e->cur_u(), scratch.ptr,
-                                   e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, e->stream);
+                                   e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, FlagOut{}, e->stream);
           (void)hipEventRecord(b, e->stream);
           (void)hipEventSynchronize(b);
           float ms = 0.f;

@@ -43,8 +43,11 @@ struct wx_ctx : Mesh, Runtime {
   DevScalars *hS = nullptr, *hS_dev = nullptr;   // mapped: k_gather_flags writes it
   int8_t *hFlag = nullptr;
   size_t hFlagN = 0;
-  // the flags by leaf position, written by k_gather_flags into mapped memory
+  // the flags by leaf position in mapped memory, written by the fill's
+  // epilogue (flags_host: they are those of the current dFlag and leaf
+  // order) or else by k_gather_flags
   int8_t *hFlagQ = nullptr, *hFlagQ_dev = nullptr;
+  bool flags_host = false;
   // cooling surrogate (coolingh only), used when use_sur
   DevBuf<SurDev> dSur;
   bool use_sur = false;
@@ -130,6 +133,7 @@ int upload_ops(wx_ctx *c) {
 // the leaf records) before the ghost list is located and compiled.
 int tables_leaves(wx_ctx *c) {
   HostTimer ht(c, HID_TABLES);
+  c->flags_host = false;   // leaf positions change
   int rc = compile_leaves(c);
   return rc ? rc : upload_leaves(c);
 }
@@ -154,6 +158,7 @@ int ensure_tables(wx_ctx *c) {
 // the last mesh change); the leaf records must be current.
 int update_stats(wx_ctx *c) {
   int rc;
+  if (c->stats_all_stale || !c->new_ids.empty()) c->flags_host = false;   // blockstat writes flags
   if (c->stats_all_stale) {
     ProfScope ps(c, KID_STAT);
     rc = launch_blockstat(c->dP.ptr, c->neq, c->tLeaves, c->nleaf(), c->cur_u(), c->dStat.ptr,
@@ -214,15 +219,17 @@ int run_stages_bf(wx_ctx *c) {
   }
   if (!rc) {
     ProfScope ps(c, KID_FILLI);
+    const FlagOut fo{c->tOff, c->hFlagQ_dev, c->hS_dev, 1};
     if (c->use_sur)
       rc = launch_fillI_cool_sur(c->dP.ptr, c->neq, c->queue(), g, 256, 0, c->tOpsI, c->tBinfo, c->cur_u(),
-                                 c->other_u(), c->dStat.ptr, c->dFlag.ptr, c->dS.ptr, c->dSur.ptr, c->stream);
+                                 c->other_u(), c->dStat.ptr, c->dFlag.ptr, c->dS.ptr, c->dSur.ptr, fo, c->stream);
     else
       rc = launch_fillI_cool(c->dP.ptr, c->neq, c->queue(), g, 256, 0, c->tOpsI, c->tBinfo, c->cur_u(),
-                             c->other_u(), c->dStat.ptr, c->dFlag.ptr, c->dS.ptr, c->stream);
+                             c->other_u(), c->dStat.ptr, c->dFlag.ptr, c->dS.ptr, fo, c->stream);
   }
   if (rc) return fail(rc, "stage launch failed: %s", hipGetErrorString(hipGetLastError()));
   c->cur ^= 1;
+  c->flags_host = true;
   c->stats_all_stale = false;   // the epilogue of the last kernel wrote every leaf's statistics
   c->new_ids.clear();
   return WX_OK;
@@ -268,9 +275,11 @@ int fetch_flags(wx_ctx *c) {
 // Riemann-failure check of sync_scalars.
 int fetch_leaf_flags(wx_ctx *c) {
   const int n = c->nleaf();
-  int rc = launch_gather_flags(c->queue(), c->tOff, std::max(1, std::min((n + 255) / 256, 1024)), c->dFlag.ptr,
-                               c->hFlagQ_dev, c->dS.ptr, c->hS_dev, 1, c->stream);
-  if (rc) return fail(rc, "flag gather launch failed");
+  if (!c->flags_host) {   // else the last fill's epilogue wrote them
+    int rc = launch_gather_flags(c->queue(), c->tOff, std::max(1, std::min((n + 255) / 256, 1024)), c->dFlag.ptr,
+                                 c->hFlagQ_dev, c->dS.ptr, c->hS_dev, 1, c->stream);
+    if (rc) return fail(rc, "flag gather launch failed");
+  }
   HIPCHK(hipStreamSynchronize(c->stream));
   if (c->hS->err) {
     int e = c->hS->err;

@@ -379,7 +379,7 @@ __global__ __launch_bounds__(BS) void k_stageB(const wx_params *__restrict__ P, 
 // stat[2id+1] = DMC t_cool min; flag[id]: 1 refine, 2 coarsen, 0 neither.
 template <int NEQ>
 __device__ void block_stats(const wx_params *__restrict__ p, const double *prim, const LeafRec &L,
-                            double *stat, int8_t *flag) {
+                            double *stat, int8_t *flag, const FlagOut &fo) {
   __shared__ double red[3][16];
   const int tid = threadIdx.x;
   const int i = tid % 16 + 1, j = tid / 16 + 1;
@@ -430,6 +430,7 @@ __device__ void block_stats(const wx_params *__restrict__ p, const double *prim,
     if (grad >= p->tol) fl = (lev < p->nlevs) ? 1 : 0;
     else if (grad <= p->tol2) fl = 2;
     flag[L.id] = fl;
+    if (fo.flagq) fo.flagq[fo.inst_off[L.inst] + L.lidx] = fl;
   }
 }
 
@@ -577,6 +578,7 @@ __global__ __launch_bounds__(BS, (SUR ? 2 : 1024 / BS)) void k_fillI_cool(const 
                                                     double *__restrict__ stat,
                                                     int8_t *__restrict__ flag,
                                                     DevScalars *__restrict__ Sa,
+                                                    FlagOut fo,
                                                     SurDev *__restrict__ M) {
   __shared__ double tile[NEQ * NC];
   // the network in LDS, staged once per workgroup (no LDS without the surrogate)
@@ -590,6 +592,10 @@ __global__ __launch_bounds__(BS, (SUR ? 2 : 1024 / BS)) void k_fillI_cool(const 
   const int b0 = PERSIST ? (int)blockIdx.x : xcd_entry(blockIdx.x, gridDim.x);
   LeafRec Lnext = q.rec[min(b0, q.cap - 1)];
   const int nleaf = q.hdr->nleaf;
+  // the scalars are final for the step (dt reduction and stages A and B
+  // are done; this kernel does not write them)
+  if (fo.flagq && blockIdx.x == 0)
+    for (int t = threadIdx.x; t < fo.ninst; t += blockDim.x) fo.Sout[t] = Sa[t];
   for (int l = b0; l < nleaf; l += gridDim.x) {
   const int r = l;
   const LeafRec L = Lnext;
@@ -675,7 +681,7 @@ __global__ __launch_bounds__(BS, (SUR ? 2 : 1024 / BS)) void k_fillI_cool(const 
     for (int k = 0; k < NEQ; ++k) tile[k * NC + IDX(i, j)] = q[k];
   }
   __syncthreads();
-  block_stats<NEQ>(p, tile, L, stat, flag);
+  block_stats<NEQ>(p, tile, L, stat, flag, fo);
   __syncthreads();   // the next queue entry reuses the LDS tile
    if (!PERSIST) break;   // one entry per workgroup: no loop
   }
@@ -703,7 +709,7 @@ __global__ __launch_bounds__(256) void k_blockstat(const wx_params *__restrict__
     for (int k = 0; k < NEQ; ++k) tile[k * NC + IDX(i, j)] = q[k];
   }
   __syncthreads();
-  block_stats<NEQ>(p, tile, L, stat, flag);
+  block_stats<NEQ>(p, tile, L, stat, flag, FlagOut{nullptr, nullptr, nullptr, 0});
 }
 
 // dt = min(cfl*dtp, 0.4*dtcool*Psc/tsc) (src/timestep.f90:50); optionally
@@ -934,19 +940,19 @@ int launch_stageB(const wx_params *P, int neq, QueueRef q, int grid, int bs, int
 
 int launch_fillI_cool(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const int2 *ops,
                       const BlockRec *binfo, const double *Uin, double *Uout, double *stat, int8_t *flag,
-                      DevScalars *S, hipStream_t stream) {
+                      DevScalars *S, FlagOut fo, hipStream_t stream) {
   if (grid <= 0) return WX_OK;
-  WX_PERSIST_DISPATCH(persist, WX_BS_DISPATCH(bs, WX_DISPATCH(neq, hipLaunchKernelGGL((k_fillI_cool<N, false, B, PS>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, nullptr))));
+  WX_PERSIST_DISPATCH(persist, WX_BS_DISPATCH(bs, WX_DISPATCH(neq, hipLaunchKernelGGL((k_fillI_cool<N, false, B, PS>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, fo, nullptr))));
   return hip_status();
 }
 
 int launch_fillI_cool_sur(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const int2 *ops,
                           const BlockRec *binfo, const double *Uin, double *Uout, double *stat, int8_t *flag,
-                          DevScalars *S, SurDev *M, hipStream_t stream) {
+                          DevScalars *S, SurDev *M, FlagOut fo, hipStream_t stream) {
   if (grid <= 0) return WX_OK;
   switch (neq) {   // coolingh needs u(5) = n_HI
-    case 5: WX_PERSIST_DISPATCH(persist, WX_BS_DISPATCH(bs, hipLaunchKernelGGL((k_fillI_cool<5, true, B, PS>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, M))); break;
-    case 6: WX_PERSIST_DISPATCH(persist, WX_BS_DISPATCH(bs, hipLaunchKernelGGL((k_fillI_cool<6, true, B, PS>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, M))); break;
+    case 5: WX_PERSIST_DISPATCH(persist, WX_BS_DISPATCH(bs, hipLaunchKernelGGL((k_fillI_cool<5, true, B, PS>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, fo, M))); break;
+    case 6: WX_PERSIST_DISPATCH(persist, WX_BS_DISPATCH(bs, hipLaunchKernelGGL((k_fillI_cool<6, true, B, PS>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, fo, M))); break;
     default: return WX_EARG;
   }
   return hip_status();

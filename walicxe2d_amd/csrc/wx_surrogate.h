@@ -63,6 +63,6 @@ int sur_upload(SurDev *M, const wx_sur_model *m, const wx_params *P, hipStream_t
 int sur_counters(SurDev *M, hipStream_t stream, int reset, int64_t *cells, int64_t *fallbacks);
 int launch_fillI_cool_sur(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const int2 *ops,
                           const BlockRec *binfo, const double *Uin, double *Uout, double *stat, int8_t *flag,
-                          DevScalars *S, SurDev *M, hipStream_t stream);
+                          DevScalars *S, SurDev *M, FlagOut fo, hipStream_t stream);
 
 }  // namespace wx
