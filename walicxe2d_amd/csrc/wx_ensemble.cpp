@@ -9,11 +9,12 @@
 // mesh change only the regions of instances whose mesh changed are
 // uploaded, and k_build_queue gathers all regions into the compacted queue.
 // Each stage is ONE launch over that queue (one workgroup per entry, or a
-// persistent grid looping over it).  The step's launches (dt reduction, the
-// three stages, the flag gather) can be captured in two hipGraphs (one per
-// tile-buffer parity); as the kernels read the queue length from device
-// memory and the buffers are allocated at full capacity up front, the
-// graphs are re-captured only when the queue outgrows the span they cover.
+// persistent grid looping over it).  The step's launches (dt reduction and
+// the three stages, the last of which also writes the flags for the host)
+// can be captured in two hipGraphs (one per tile-buffer parity); as the
+// kernels read the queue length from device memory and the buffers are
+// allocated at full capacity up front, the graphs are re-captured only when
+// the queue outgrows the span they cover.
 // Per-instance host work (table compilation, admesh) runs on a pool of
 // worker threads; wx_ens_create_groups splits the instances into groups
 // (each of the above per group) stepped concurrently from host threads.
@@ -229,8 +230,9 @@ int ensure_stats(EnsGroup *e) {
   return WX_OK;
 }
 
-// dt reduction (time += dt), the three stages and the flag gather, reading
-// U[par] and writing U[par ^ 1]; `prof` records kernel events.
+// dt reduction (time += dt) and the three stages (the fill also writing the
+// flags and scalars for the host), reading U[par] and writing U[par ^ 1];
+// `prof` records kernel events.
 int launch_step(EnsGroup *e, int par, bool for_graph) {
   const bool prof = !for_graph;
   double *Ucur = e->U[par].ptr, *Uoth = e->U[par ^ 1].ptr;
