@@ -103,9 +103,22 @@ int launch_gather_flags(QueueRef q, const int32_t *inst_off, int, const int8_t *
   if (Sout) for (int i = 0; i < ninst; ++i) Sout[i] = S[i];
   return 0;
 }
-int launch_meshops(int, const MeshOp *ops, int nops, double *, hipStream_t) {
+int launch_meshops(const wx_params *P, int, const MeshOp *ops, int nops, double *, double *stat, int8_t *flag,
+                   hipStream_t) {
   std::lock_guard<std::mutex> g(g_mu);
-  for (int k = 0; k < nops; ++k) g_ops.push_back(ops[k]);
+  for (int k = 0; k < nops; ++k) {
+    g_ops.push_back(ops[k]);
+    if (!stat) continue;
+    LeafRec r[4] = {};
+    const int ids[4] = {ops[k].s1, ops[k].s2, ops[k].s3, ops[k].s4};
+    const int n = ops[k].kind == 0 ? 4 : 1;
+    for (int s = 0; s < n; ++s) {
+      r[s].id = ops[k].kind == 0 ? ids[s] : ops[k].a;
+      r[s].level = ops[k].level;
+      r[s].inst = ops[k].inst;
+    }
+    write_flags(P, r, n, (signed char *)flag, stat);
+  }
   return 0;
 }
 int launch_transpose(int, int, const double *, double *, int, int, hipStream_t) { return 0; }

@@ -80,7 +80,7 @@ struct DevScalars {
 struct MeshOp {
   int32_t kind;   // 0 refine (a = parent), 1 coarse (a = dad)
   int32_t a, s1, s2, s3, s4;
-  int32_t pad[2];
+  int32_t level, inst;   // level and instance of the new leaves (sons, or the dad)
 };
 
 // Where the fill's epilogue also puts the refinement flags and the scalars
@@ -131,7 +131,10 @@ int launch_dtreduce(const wx_params *P, const LeafRec *rec, const double *stat, 
 // flags by leaf position, whatever order the queue holds); Sout = S.
 int launch_gather_flags(QueueRef q, const int32_t *inst_off, int grid, const int8_t *flag, int8_t *flagq,
                         const DevScalars *S, DevScalars *Sout, int ninst, hipStream_t stream);
-int launch_meshops(int neq, const MeshOp *ops, int nops, double *U, hipStream_t stream);
+// The data parts of refine/coarse; with stat != nullptr also the block
+// statistics of the new leaves (as k_blockstat would compute them).
+int launch_meshops(const wx_params *P, int neq, const MeshOp *ops, int nops, double *U, double *stat, int8_t *flag,
+                   hipStream_t stream);
 // Performs the n copies described in d (itself in mapped host memory), one
 // launch for all: a kernel reads the pinned staging directly.
 int launch_h2d(const CopyDesc *d, int n, hipStream_t stream);

@@ -310,7 +310,10 @@ int admesh_all(EnsGroup *e, int *changed) {
   e->ops.clear();
   for (int i = 0; i < e->n; ++i) {
     const int sb = e->slotbase[i];
+    const Mesh &m = e->m[i];
     for (MeshOp o : e->iops[i]) {
+      o.level = m.LP(o.kind == 0 ? o.s1 : o.a, 1);   // the new leaves', for their statistics
+      o.inst = i;
       o.a += sb; o.s1 += sb; o.s2 += sb; o.s3 += sb; o.s4 += sb;
       e->ops.push_back(o);
     }
@@ -322,8 +325,11 @@ int admesh_all(EnsGroup *e, int *changed) {
   if ((rc = stage_copy(e, e->dMeshOps.ptr, e->ops.data(), e->ops.size() * sizeof(MeshOp)))) return rc;
   if ((rc = stage_end(e))) return rc;
   ProfScope ps(e, KID_MESH);
-  rc = launch_meshops(e->neq, e->dMeshOps.ptr, (int)e->ops.size(), e->cur_u(), e->stream);
+  rc = launch_meshops(e->dP.ptr, e->neq, e->dMeshOps.ptr, (int)e->ops.size(), e->cur_u(), e->dStat.ptr, e->dFlag.ptr,
+                      e->stream);
   if (rc) return fail(rc, "meshops launch failed");
+  for (int i = 0; i < e->n; ++i)   // the kernel computed the new leaves' statistics
+    if (!e->iops[i].empty()) e->m[i].new_ids.clear();
   return WX_OK;
 }
 

@@ -240,8 +240,16 @@ int run_stages(wx_ctx *c) {
   return rc ? rc : run_stages_bf(c);
 }
 
-int run_meshops(wx_ctx *c, const std::vector<MeshOp> &ops) {
+// The data ops of a mesh change; with `stats` (the ops of this context's own
+// admesh, whose tree already holds the new leaves) the kernel also computes
+// the new leaves' statistics, so they need no k_blockstat.
+int run_meshops(wx_ctx *c, std::vector<MeshOp> &ops, bool stats) {
   if (ops.empty()) return WX_OK;
+  if (stats)
+    for (MeshOp &o : ops) {
+      o.level = c->LP(o.kind == 0 ? o.s1 : o.a, 1);
+      o.inst = 0;
+    }
   int rc = c->dMeshOps.ensure(ops.size());
   if (rc) return rc;
   if ((rc = stage_begin(c, al256(ops.size() * sizeof(MeshOp))))) return rc;
@@ -249,9 +257,11 @@ int run_meshops(wx_ctx *c, const std::vector<MeshOp> &ops) {
   if ((rc = stage_end(c))) return rc;
   {
     ProfScope ps(c, KID_MESH);
-    rc = launch_meshops(c->neq, c->dMeshOps.ptr, (int)ops.size(), c->cur_u(), c->stream);
+    rc = launch_meshops(c->dP.ptr, c->neq, c->dMeshOps.ptr, (int)ops.size(), c->cur_u(), stats ? c->dStat.ptr : nullptr,
+                        stats ? c->dFlag.ptr : nullptr, c->stream);
   }
   if (rc) return fail(rc, "meshops launch failed");
+  if (stats) c->new_ids.clear();   // the new leaves' statistics are done
   return WX_OK;
 }
 
@@ -298,7 +308,7 @@ int do_admesh(wx_ctx *c, int *changed) {
   rc = admesh_from_flags(c, c->hFlagQ, ops);
   if (rc) return rc;
   if (changed) *changed = (int)ops.size();
-  return run_meshops(c, ops);
+  return run_meshops(c, ops, true);
 }
 
 }  // namespace
@@ -501,20 +511,20 @@ int wx_refine(wx_ctx *c, int32_t nb, int32_t s1, int32_t s2, int32_t s3, int32_t
   if (!c) return fail(WX_EARG, "null context");
   const int ids[5] = {nb, s1, s2, s3, s4};
   for (int k = 0; k < 5; ++k) if (ids[k] < 1 || ids[k] > c->cap) return fail(WX_EARG, "refine: block %d out of range", ids[k]);
-  std::vector<MeshOp> ops{MeshOp{0, nb, s1, s2, s3, s4, {0, 0}}};
+  std::vector<MeshOp> ops{MeshOp{0, nb, s1, s2, s3, s4, 0, 0}};
   c->stats_all_stale = true;
   c->tables_valid = false;
-  return run_meshops(c, ops);
+  return run_meshops(c, ops, false);
 }
 
 int wx_coarse(wx_ctx *c, int32_t dad, int32_t s1, int32_t s2, int32_t s3, int32_t s4) {
   if (!c) return fail(WX_EARG, "null context");
   const int ids[5] = {dad, s1, s2, s3, s4};
   for (int k = 0; k < 5; ++k) if (ids[k] < 1 || ids[k] > c->cap) return fail(WX_EARG, "coarse: block %d out of range", ids[k]);
-  std::vector<MeshOp> ops{MeshOp{1, dad, s1, s2, s3, s4, {0, 0}}};
+  std::vector<MeshOp> ops{MeshOp{1, dad, s1, s2, s3, s4, 0, 0}};
   c->stats_all_stale = true;
   c->tables_valid = false;
-  return run_meshops(c, ops);
+  return run_meshops(c, ops, false);
 }
 
 int wx_basegrid(wx_ctx *c, const double *uroots, int32_t nroots) {

@@ -796,9 +796,12 @@ __global__ __launch_bounds__(256) void k_gather_flags(QueueRef q, const int32_t 
 // and src/coarse.f90:182-366 (MPIP branch: 2x2 means incl. ghosts, then
 // outer ghost copies).  One workgroup per op.
 template <int NEQ>
-__global__ __launch_bounds__(256) void k_meshops(const MeshOp *__restrict__ ops, double *__restrict__ U) {
+__global__ __launch_bounds__(256) void k_meshops(const wx_params *__restrict__ P, const MeshOp *__restrict__ ops,
+                                                 double *__restrict__ U, double *__restrict__ stat,
+                                                 int8_t *__restrict__ flag) {
   const MeshOp op = ops[blockIdx.x];
   const int tid = threadIdx.x;
+  __shared__ double d[NEQ * NC];   // coarse: the dad tile; then a tile of primitives for the statistics
   if (op.kind == 0) {
     const double *pb = U + (size_t)op.a * NEQ * NC;
     const int sons[4] = {op.s1, op.s2, op.s3, op.s4};
@@ -811,8 +814,31 @@ __global__ __launch_bounds__(256) void k_meshops(const MeshOp *__restrict__ ops,
 #pragma unroll
       for (int k = 0; k < NEQ; ++k) sb[k * NC + cc] = pb[k * NC + IDX(si, sj)];
     }
+    if (!stat) return;
+    // statistics of each son from its cells 0..17, which are the parent's
+    // cells it was injected from (read back from the parent, not the son)
+    const wx_params *__restrict__ p = P + op.inst;
+    for (int s = 0; s < 4; ++s) {
+      for (int c = tid; c < 18 * 18; c += blockDim.x) {
+        const int i = c % 18, j = c / 18;
+        const int si = (i + 1) / 2 + ((s & 1) ? NX / 2 : 0);
+        const int sj = (j + 1) / 2 + ((s & 2) ? NY / 2 : 0);
+        double uu[NEQ], q[NEQ], T;
+#pragma unroll
+        for (int k = 0; k < NEQ; ++k) uu[k] = pb[k * NC + IDX(si, sj)];
+        uprim<NEQ>(p, q, uu, T);
+#pragma unroll
+        for (int k = 0; k < NEQ; ++k) d[k * NC + IDX(i, j)] = q[k];
+      }
+      __syncthreads();
+      LeafRec L{};
+      L.id = sons[s];
+      L.level = op.level;
+      L.inst = op.inst;
+      block_stats<NEQ>(p, d, L, stat, flag, FlagOut{nullptr, nullptr, nullptr, 0});
+      __syncthreads();   // d is refilled for the next son
+    }
   } else {
-    __shared__ double d[NEQ * NC];
     const int sons[4] = {op.s1, op.s2, op.s3, op.s4};
     // start from the current dad tile (corners the rules below do not set)
     const double *dbg = U + (size_t)op.a * NEQ * NC;
@@ -847,6 +873,25 @@ __global__ __launch_bounds__(256) void k_meshops(const MeshOp *__restrict__ ops,
     __syncthreads();
     double *db = U + (size_t)op.a * NEQ * NC;
     for (int c = tid; c < NEQ * NC; c += blockDim.x) db[c] = d[c];
+    if (!stat) return;
+    // statistics of the dad: primitives of its cells 0..17 in place
+    const wx_params *__restrict__ p = P + op.inst;
+    __syncthreads();
+    for (int c = tid; c < 18 * 18; c += blockDim.x) {
+      const int i = c % 18, j = c / 18;
+      double uu[NEQ], q[NEQ], T;
+#pragma unroll
+      for (int k = 0; k < NEQ; ++k) uu[k] = d[k * NC + IDX(i, j)];
+      uprim<NEQ>(p, q, uu, T);
+#pragma unroll
+      for (int k = 0; k < NEQ; ++k) d[k * NC + IDX(i, j)] = q[k];
+    }
+    __syncthreads();
+    LeafRec L{};
+    L.id = op.a;
+    L.level = op.level;
+    L.inst = op.inst;
+    block_stats<NEQ>(p, d, L, stat, flag, FlagOut{nullptr, nullptr, nullptr, 0});
   }
 }
 
@@ -980,9 +1025,10 @@ int launch_gather_flags(QueueRef q, const int32_t *inst_off, int grid, const int
   return hip_status();
 }
 
-int launch_meshops(int neq, const MeshOp *ops, int nops, double *U, hipStream_t stream) {
+int launch_meshops(const wx_params *P, int neq, const MeshOp *ops, int nops, double *U, double *stat, int8_t *flag,
+                   hipStream_t stream) {
   if (nops <= 0) return WX_OK;
-  WX_DISPATCH(neq, hipLaunchKernelGGL(k_meshops<N>, dim3(nops), dim3(256), 0, stream, ops, U));
+  WX_DISPATCH(neq, hipLaunchKernelGGL(k_meshops<N>, dim3(nops), dim3(256), 0, stream, P, ops, U, stat, flag));
   return hip_status();
 }
 

@@ -315,7 +315,7 @@ static int admesh_tree(Mesh *c, std::vector<uint8_t> &up, std::vector<uint8_t> &
         s1 = c->LP(nb, 3); s2 = c->LP(s1, 6); s3 = c->LP(s1, 8); s4 = c->LP(s3, 6);
       }
       c->nbmax = nbmaxp;
-      ops.push_back(MeshOp{0, nb, s1, s2, s3, s4, {0, 0}});
+      ops.push_back(MeshOp{0, nb, s1, s2, s3, s4, 0, 0});
       updatelpup(c, nb, s1, s2, s3, s4, up, dn);
     }
   }
@@ -334,7 +334,7 @@ static int admesh_tree(Mesh *c, std::vector<uint8_t> &up, std::vector<uint8_t> &
       if (!(dn[nb] && c->LP(nb, 4) == 1 && c->LP(nb, 1) == nl)) continue;
       const int dad = c->LP(nb, 2), s1 = nb, s2 = c->LP(nb, 6), s3 = c->LP(nb, 8), s4 = c->LP(c->LP(nb, 6), 8);
       if (dn[s2] && dn[s3] && dn[s4]) {
-        ops.push_back(MeshOp{1, dad, s1, s2, s3, s4, {0, 0}});
+        ops.push_back(MeshOp{1, dad, s1, s2, s3, s4, 0, 0});
         updatelpdown(c, dad, s1, s2, s3, s4);
       }
     }
