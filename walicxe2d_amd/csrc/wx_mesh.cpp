@@ -48,26 +48,34 @@ void Mesh::init(const wx_params &pp) {
 // ---------------------------------------------------------------------
 // src/locatebounds.f90:29-540, MPIP forms, one rank.
 void locatebounds(Mesh *c) {
-  std::vector<uint8_t> bnd((size_t)(c->nbmax + 1) * 4, 0);
-  c->bounds.clear();
-  static const int opp[4] = {1, 0, 3, 2};
+  const int nbmax = c->nbmax;
+  std::vector<uint8_t> &bnd = c->bndDone;   // face already in the list
+  bnd.assign((size_t)(nbmax + 1) * 4, 0);
+  std::vector<int32_t> &out = c->bounds;
+  out.resize((size_t)c->nbleafs * 4 * 3 + 3);   // 4 entries per leaf; add() grows it beyond
+  size_t n = 0;
+  const int32_t *lp = c->lp.data(), *leaf = c->leaf.data();
+  auto LP = [lp](int nb, int k) { return lp[(size_t)nb * 8 + k - 1]; };
   auto add = [&](int a, int b, int code) {
-    c->bounds.push_back(a);
-    c->bounds.push_back(b);
-    c->bounds.push_back(code);
+    if (n + 3 > out.size()) out.resize(out.size() * 2 + 12);
+    out[n] = a;
+    out[n + 1] = b;
+    out[n + 2] = code;
+    n += 3;
   };
-  for (int nb = 1; nb <= c->nbmax; ++nb) {
-    if (!c->leaf[nb]) continue;
+  static const int opp[4] = {1, 0, 3, 2};
+  for (int nb = 1; nb <= nbmax; ++nb) {
+    if (!leaf[nb]) continue;
     for (int f = 0; f < 4; ++f) {
       if (bnd[(size_t)nb * 4 + f]) continue;
-      const int nbr = c->LP(nb, 5 + f);
+      const int nbr = LP(nb, 5 + f);
       if (nbr == -1) {
         bnd[(size_t)nb * 4 + f] = 1;
         add(nb, 0, -(f + 1));
         continue;
       }
-      if (c->LP(nbr, 1) < c->LP(nb, 1)) continue;   // coarser side owns the face
-      if (c->leaf[nbr]) {
+      if (LP(nbr, 1) < LP(nb, 1)) continue;   // coarser side owns the face
+      if (leaf[nbr]) {
         add(nb, nbr, 1 + 3 * f);
         bnd[(size_t)nb * 4 + f] = 1;
         bnd[(size_t)nbr * 4 + opp[f]] = 1;
@@ -75,17 +83,18 @@ void locatebounds(Mesh *c) {
         bnd[(size_t)nb * 4 + f] = 1;
         int n1, n2;
         switch (f) {
-          case 0: n1 = c->LP(c->LP(nbr, 3), 6); n2 = c->LP(n1, 8); break;   // :130-131
-          case 1: n1 = c->LP(nbr, 3); n2 = c->LP(n1, 8); break;             // :237-238
-          case 2: n1 = c->LP(c->LP(nbr, 3), 8); n2 = c->LP(n1, 6); break;   // :345-346
-          default: n1 = c->LP(nbr, 3); n2 = c->LP(n1, 6); break;            // :452-453
+          case 0: n1 = LP(LP(nbr, 3), 6); n2 = LP(n1, 8); break;   // :130-131
+          case 1: n1 = LP(nbr, 3); n2 = LP(n1, 8); break;          // :237-238
+          case 2: n1 = LP(LP(nbr, 3), 8); n2 = LP(n1, 6); break;   // :345-346
+          default: n1 = LP(nbr, 3); n2 = LP(n1, 6); break;         // :452-453
         }
         add(nb, n1, 2 + 3 * f);
         add(nb, n2, 3 + 3 * f);
       }
     }
   }
-  c->ninternal = (int)(c->bounds.size() / 3);
+  out.resize(n);
+  c->ninternal = (int)(n / 3);
 }
 
 // ---------------------------------------------------------------------
@@ -93,22 +102,30 @@ void locatebounds(Mesh *c) {
 // position of every block ID and the block records, from the tree.
 int compile_leaves(Mesh *c) {
   const int cap = c->nbmax;   // IDs above nbmax hold no block
-  c->leaves.clear();
   c->lidx.assign((size_t)cap + 1, -1);
   c->binfo.resize((size_t)cap + 1);
-  c->binfo[0] = BlockRec{0, 0, 0, 0};
-  for (int nb = 1; nb <= c->nbmax; ++nb) {
-    c->binfo[nb] = BlockRec{c->LP(nb, 1), c->icoord[(size_t)nb * 2], c->icoord[(size_t)nb * 2 + 1], 0};
-    if (!c->leaf[nb]) continue;
-    c->lidx[nb] = (int)c->leaves.size();
+  const int32_t *lp = c->lp.data(), *leaf = c->leaf.data(), *ic = c->icoord.data();
+  int *lidx = c->lidx.data();
+  BlockRec *bi = c->binfo.data();
+  bi[0] = BlockRec{0, 0, 0, 0};
+  int nleaf = 0;
+  for (int nb = 1; nb <= cap; ++nb) {
+    bi[nb] = BlockRec{lp[(size_t)nb * 8], ic[(size_t)nb * 2], ic[(size_t)nb * 2 + 1], 0};
+    if (leaf[nb]) lidx[nb] = nleaf++;
+  }
+  c->leaves.resize((size_t)nleaf);
+  LeafRec *rec = c->leaves.data();
+  for (int nb = 1; nb <= cap; ++nb) {
+    const int l = lidx[nb];
+    if (l < 0) continue;
     LeafRec r{};
     r.id = nb;
-    r.level = c->LP(nb, 1);
-    r.icx = c->icoord[(size_t)nb * 2];
-    r.icy = c->icoord[(size_t)nb * 2 + 1];
-    r.lidx = (int)c->leaves.size();
+    r.level = bi[nb].level;
+    r.icx = bi[nb].icx;
+    r.icy = bi[nb].icy;
+    r.lidx = l;
     if (r.level < 1 || r.level > c->p.nlevs) return fail(WX_EARG, "block %d has level %d", nb, r.level);
-    c->leaves.push_back(r);
+    rec[l] = r;
   }
   return WX_OK;
 }
@@ -123,46 +140,55 @@ int compile_ops(Mesh *c) {
   const int cap = c->nbmax;
   const int nleaf = (int)c->leaves.size();
   const size_t nseg = (size_t)nleaf * 8;
-  c->opsII.assign(nseg, int2{0, GK_NONE});
-  c->opsI.assign(nseg, int2{0, GK_NONE});
   const bool otherb = c->p.otherb != 0;
-  if (otherb) {
-    c->segE2.resize(nseg);
+  const bool outfyb_bug = (c->p.compat & 2) && c->p.bc[2] == 0;
+  // Fills I and II receive the same writes except for the OUTFYB entries of
+  // boundaryI (src/boundaryI.f90:87: a bottom box entry writes block e + 1
+  // instead of its own block): fill II is compiled from the list without
+  // those entries, fill I starts as its copy, and then each of those entries
+  // is applied where it is the last writer (list position of every
+  // segment's writer kept in segE2 / segE1).
+  const bool track = otherb || outfyb_bug;
+  c->opsII.assign(nseg, int2{0, GK_NONE});
+  c->opsI.resize(nseg);
+  if (track) {
+    c->segE2.assign(nseg, -1);
     c->segE1.resize(nseg);
+  }
+  if (otherb) {
     c->ifirst.assign((size_t)cap + 1, INT_MAX);
     c->ilast.assign((size_t)cap + 1, -1);
   }
   int2 *o2 = c->opsII.data(), *o1 = c->opsI.data();
-  int *e2 = otherb ? c->segE2.data() : nullptr, *e1 = otherb ? c->segE1.data() : nullptr;
+  int *e2 = track ? c->segE2.data() : nullptr, *e1 = track ? c->segE1.data() : nullptr;
   const int *lidx = c->lidx.data();
-  auto put = [&](int2 *T, int *E, int dst, int side, int half, int kind, int src, int off, int e) {
-    if (dst < 1 || dst > cap) return;
-    const int li = lidx[dst];
-    if (li < 0) return;   // not a leaf: its data is never read again
-    const size_t k = (size_t)li * 8 + side * 2 + half;
-    T[k] = int2{src, kind | (off << GF_OFF_SHIFT)};
-    if (E) E[k] = e;
+  auto seg = [&](int dst, int side, int half) -> long {   // -1: not a leaf (its data is never read again)
+    if (dst < 1 || dst > cap || lidx[dst] < 0) return -1;
+    return (long)lidx[dst] * 8 + side * 2 + half;
+  };
+  auto put = [&](int dst, int side, int half, int kind, int src, int off, int e) {
+    const long k = seg(dst, side, half);
+    if (k < 0) return;
+    o2[k] = int2{src, kind | (off << GF_OFF_SHIFT)};
+    if (e2) e2[k] = e;
   };
   auto both = [&](int dst, int side, int kind, int src, int off, int e) {
-    for (int h = 0; h < 2; ++h) {
-      put(o2, e2, dst, side, h, kind, src, off, e);
-      put(o1, e1, dst, side, h, kind, src, off, e);
-    }
+    put(dst, side, 0, kind, src, off, e);
+    put(dst, side, 1, kind, src, off, e);
   };
-  const bool outfyb_bug = (c->p.compat & 2) && c->p.bc[2] == 0;
+  std::vector<int> &late = c->segLate;   // the OUTFYB entries, in list order
+  late.clear();
   for (int e = 0; e < c->ninternal; ++e) {
     const int nbs = c->bounds[(size_t)e * 3], n1 = c->bounds[(size_t)e * 3 + 1], code = c->bounds[(size_t)e * 3 + 2];
     if (nbs < 1 || nbs > c->nbmax) return fail(WX_EARG, "bound %d: block %d out of range", e + 1, nbs);
     if (code < 0) {
       const int side = -code - 1;
       if (side > 3) return fail(WX_EARG, "bound %d: code %d", e + 1, code);
-      const int kind = c->p.bc[side] ? GK_BOXREF : GK_BOXOUT;
-      for (int h = 0; h < 2; ++h) put(o2, e2, nbs, side, h, kind, nbs, 0, e);   // boundaryII.f90:38-141
       if (side == 2 && outfyb_bug) {
-        const int k = e + 1;                                                    // boundaryI.f90:87
-        for (int h = 0; h < 2; ++h) put(o1, e1, k, 2, h, GK_BOXOUT, k, 0, e);
+        late.push_back(e);
       } else {
-        for (int h = 0; h < 2; ++h) put(o1, e1, nbs, side, h, kind, nbs, 0, e);
+        const int kind = c->p.bc[side] ? GK_BOXREF : GK_BOXOUT;
+        both(nbs, side, kind, nbs, 0, e);   // boundaryII.f90:38-141, boundaryI.f90:38-107
       }
     } else {
       if (n1 < 1 || n1 > c->nbmax) return fail(WX_EARG, "bound %d: neighbour %d out of range", e + 1, n1);
@@ -175,8 +201,7 @@ int compile_ops(Mesh *c) {
           const int side = (code - 2) / 3;       // 0 L, 1 R, 2 B, 3 T
           const int h = (code - 2) % 3;          // 0: first fine block, 1: second
           const int fine_side = side ^ 1;        // the fine block's facing side
-          put(o2, e2, nbs, side, h, GK_RESTRICT, n1, 0, e);
-          put(o1, e1, nbs, side, h, GK_RESTRICT, n1, 0, e);
+          put(nbs, side, h, GK_RESTRICT, n1, 0, e);
           both(n1, fine_side, GK_PROLONG, nbs, 8 * h, e);
           break;
         }
@@ -188,30 +213,48 @@ int compile_ops(Mesh *c) {
       c->ilast[nbs] = std::max(c->ilast[nbs], e);
     }
   }
+  const int *ifirst = otherb ? c->ifirst.data() : nullptr, *ilast = otherb ? c->ilast.data() : nullptr;
+  // imposition flags of the writer of segment k of the leaf with ID dst
+  auto impose = [&](int2 &op, int e, int dst) {
+    if (ifirst[op.x] < e) op.y |= GF_SRC_IMPOSED;
+    if (ilast[dst] >= e) op.y |= GF_DST_IMPOSED;
+  };
+  if (otherb)
+    for (int l = 0; l < nleaf; ++l) {
+      const int dst = c->leaves[l].id;
+      for (int s = 0; s < 8; ++s) {
+        const size_t k = (size_t)l * 8 + s;
+        if ((o2[k].y & 15) != GK_NONE) impose(o2[k], e2[k], dst);
+      }
+      c->leaves[l].flags = ilast[dst] >= 0 ? 1 : 0;
+    }
+  memcpy(o1, o2, nseg * sizeof(int2));
+  if (track) memcpy(e1, e2, nseg * sizeof(int));
+  for (int e : late) {
+    const int nbs = c->bounds[(size_t)e * 3];
+    const int kind = c->p.bc[2] ? GK_BOXREF : GK_BOXOUT;
+    for (int h = 0; h < 2; ++h) {
+      const long k2 = seg(nbs, 2, h);             // fill II: the block's own bottom ghosts
+      if (k2 >= 0 && e > e2[k2]) {
+        o2[k2] = int2{nbs, kind};
+        e2[k2] = e;
+        if (otherb) impose(o2[k2], e, nbs);
+      }
+      const int t = e + 1;                        // fill I: block e + 1 (src/boundaryI.f90:87)
+      const long k1 = seg(t, 2, h);
+      if (k1 >= 0 && e > e1[k1]) {
+        o1[k1] = int2{t, GK_BOXOUT};
+        e1[k1] = e;
+        if (otherb) impose(o1[k1], e, t);
+      }
+    }
+  }
   // every segment of fill II is written by the list (boundaryI may leave
   // segments to the carried values)
   for (size_t k = 0; k < nseg; ++k)
     if ((o2[k].y & 15) == GK_NONE)
       return fail(WX_EARG, "boundaryII: block %d ghost segment %d not covered by the list",
                   c->leaves[k / 8].id, (int)(k % 8));
-  if (otherb) {
-    const int *ifirst = c->ifirst.data(), *ilast = c->ilast.data();
-    for (int l = 0; l < nleaf; ++l) {
-      const int dst = c->leaves[l].id;
-      for (int s = 0; s < 8; ++s) {
-        const size_t k = (size_t)l * 8 + s;
-        if ((o2[k].y & 15) != GK_NONE) {
-          if (ifirst[o2[k].x] < e2[k]) o2[k].y |= GF_SRC_IMPOSED;
-          if (ilast[dst] >= e2[k]) o2[k].y |= GF_DST_IMPOSED;
-        }
-        if ((o1[k].y & 15) != GK_NONE) {
-          if (ifirst[o1[k].x] < e1[k]) o1[k].y |= GF_SRC_IMPOSED;
-          if (ilast[dst] >= e1[k]) o1[k].y |= GF_DST_IMPOSED;
-        }
-      }
-      c->leaves[l].flags = ilast[dst] >= 0 ? 1 : 0;
-    }
-  }
   return WX_OK;
 }
 

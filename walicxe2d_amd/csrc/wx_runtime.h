@@ -66,13 +66,14 @@ struct Mesh {
   std::vector<int32_t> lp, leaf, icoord;   // lp[nb*8 + k-1], leaf[nb], icoord[nb*2 + d]
   int nbmax = 0, nbleafs = 0;
   std::vector<int32_t> bounds;   // the ordered ghost list, [e*3 + c]
+  std::vector<uint8_t> bndDone;  // locatebounds scratch
   int ninternal = 0;
   // gather tables compiled from the list
   std::vector<LeafRec> leaves;
   std::vector<BlockRec> binfo;
   std::vector<int2> opsII, opsI;
   std::vector<int> lidx;
-  std::vector<int> segE2, segE1, ifirst, ilast;   // compile_tables scratch
+  std::vector<int> segE2, segE1, ifirst, ilast, segLate;   // compile_ops scratch
   bool tables_valid = false;     // leaves / binfo / ops match the tree and the list
   bool bounds_external = false;  // the list came from the host (wx_set_bounds)
   // block statistics (CFL partials + refinement flag) are kept per block ID;
