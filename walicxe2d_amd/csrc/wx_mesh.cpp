@@ -362,19 +362,30 @@ static int admesh_tree(Mesh *c, std::vector<uint8_t> &up, std::vector<uint8_t> &
       updatelpup(c, nb, s1, s2, s3, s4, up, dn);
     }
   }
-  std::vector<int> cand;
-  for (int nb : dnl) {                               // critneighdown, src/critneighdown.f90:14-50
-    if (!dn[nb]) continue;
-    for (int f = 5; f <= 8; ++f) {
-      const int n1 = c->LP(nb, f);
-      if (n1 != -1 && !c->leaf[n1]) dn[nb] = 0;
+  // critneighdown, src/critneighdown.f90:14-50: candidates (first sons that
+  // keep their flag) by level, each level in ascending ID order.  The
+  // coarsening below changes neither levels, sibling numbers nor flags, so
+  // the candidates of every level are known before it starts.
+  std::vector<std::vector<int>> cand((size_t)nlevs + 2);
+  {
+    const int32_t *lp = c->lp.data(), *leaf = c->leaf.data();
+    uint8_t *dnp = dn.data();
+    std::vector<int> &dl = dnl;
+    if (!std::is_sorted(dl.begin(), dl.end())) std::sort(dl.begin(), dl.end());
+    for (int nb : dl) {
+      if (!dnp[nb]) continue;
+      const int32_t *L = lp + (size_t)nb * 8;
+      bool keep = true;
+      for (int f = 4; f < 8; ++f) {
+        const int n1 = L[f];
+        if (n1 != -1 && !leaf[n1]) keep = false;
+      }
+      if (!keep) dnp[nb] = 0;
+      else if (nb <= nbmaxp1 && L[3] == 1 && L[0] >= 2 && L[0] <= nlevs) cand[L[0]].push_back(nb);
     }
-    if (dn[nb] && nb <= nbmaxp1 && c->LP(nb, 4) == 1) cand.push_back(nb);
   }
-  std::sort(cand.begin(), cand.end());
   for (int nl = nlevs; nl >= 2; --nl)                // coarsen by level, src/admesh.f90:75-105
-    for (int nb : cand) {
-      if (!(dn[nb] && c->LP(nb, 4) == 1 && c->LP(nb, 1) == nl)) continue;
+    for (int nb : cand[nl]) {
       const int dad = c->LP(nb, 2), s1 = nb, s2 = c->LP(nb, 6), s3 = c->LP(nb, 8), s4 = c->LP(c->LP(nb, 6), 8);
       if (dn[s2] && dn[s3] && dn[s4]) {
         ops.push_back(MeshOp{1, dad, s1, s2, s3, s4, 0, 0});
