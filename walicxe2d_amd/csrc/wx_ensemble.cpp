@@ -53,8 +53,6 @@ struct EnsGroup : Runtime {
   DevBuf<int2> dRegOpsII, dRegOpsI, dOpsII, dOpsI;
   DevBuf<double> dStat;
   DevBuf<int8_t> dFlag;
-  DevBuf<MeshOp> dMeshOps;
-  DevBuf<LeafRec> dDirty;
   // mapped pinned mirrors written by the fill's epilogue (or k_gather_flags)
   DevScalars *hS = nullptr, *hS_dev = nullptr;
   int8_t *hFlagQ = nullptr, *hFlagQ_dev = nullptr;
@@ -219,15 +217,15 @@ int ensure_stats(EnsGroup *e) {
     m.new_ids.clear();
   }
   if (e->dirty.empty()) return WX_OK;
-  if ((rc = e->dDirty.ensure(e->dirty.size()))) return rc;
-  if ((rc = stage_begin(e, al256(e->dirty.size() * sizeof(LeafRec))))) return rc;
-  if ((rc = stage_copy(e, e->dDirty.ptr, e->dirty.data(), e->dirty.size() * sizeof(LeafRec)))) return rc;
-  if ((rc = stage_end(e))) return rc;
-  ProfScope ps(e, KID_STAT);
-  rc = launch_blockstat(e->dP.ptr, e->neq, e->dDirty.ptr, (int)e->dirty.size(), e->cur_u(), e->dStat.ptr,
-                        e->dFlag.ptr, e->stream);
+  const LeafRec *drec = (const LeafRec *)stage_inplace(e, e->dirty.data(), e->dirty.size() * sizeof(LeafRec));
+  if (!drec) return WX_ENOMEM;
+  {
+    ProfScope ps(e, KID_STAT);
+    rc = launch_blockstat(e->dP.ptr, e->neq, drec, (int)e->dirty.size(), e->cur_u(), e->dStat.ptr, e->dFlag.ptr,
+                          e->stream);
+  }
   if (rc) return fail(rc, "blockstat launch failed");
-  return WX_OK;
+  return stage_end(e);
 }
 
 // dt reduction (time += dt) and the three stages (the fill also writing the
@@ -320,14 +318,16 @@ int admesh_all(EnsGroup *e, int *changed) {
   }
   if (changed) *changed = (int)e->ops.size();
   if (e->ops.empty()) return WX_OK;
-  if ((rc = e->dMeshOps.ensure(e->ops.size()))) return rc;
-  if ((rc = stage_begin(e, al256(e->ops.size() * sizeof(MeshOp))))) return rc;
-  if ((rc = stage_copy(e, e->dMeshOps.ptr, e->ops.data(), e->ops.size() * sizeof(MeshOp)))) return rc;
-  if ((rc = stage_end(e))) return rc;
-  ProfScope ps(e, KID_MESH);
-  rc = launch_meshops(e->dP.ptr, e->neq, e->dMeshOps.ptr, (int)e->ops.size(), e->cur_u(), e->dStat.ptr, e->dFlag.ptr,
-                      e->stream);
+  // the kernel reads the op list in place from the pinned staging
+  const MeshOp *dops = (const MeshOp *)stage_inplace(e, e->ops.data(), e->ops.size() * sizeof(MeshOp));
+  if (!dops) return WX_ENOMEM;
+  {
+    ProfScope ps(e, KID_MESH);
+    rc = launch_meshops(e->dP.ptr, e->neq, dops, (int)e->ops.size(), e->cur_u(), e->dStat.ptr, e->dFlag.ptr,
+                        e->stream);
+  }
   if (rc) return fail(rc, "meshops launch failed");
+  if ((rc = stage_end(e))) return rc;
   for (int i = 0; i < e->n; ++i)   // the kernel computed the new leaves' statistics
     if (!e->iops[i].empty()) e->m[i].new_ids.clear();
   return WX_OK;
@@ -418,7 +418,7 @@ int grp_destroy(EnsGroup *e) {
   e->dP.release(); e->dS.release(); e->dQ.release(); e->dInstOff.release(); e->dLeafBase.release();
   e->dRegRec.release(); e->dRec.release(); e->dBinfo.release(); e->dRegOpsII.release(); e->dRegOpsI.release();
   e->dOpsII.release(); e->dOpsI.release();
-  e->dStat.release(); e->dFlag.release(); e->dMeshOps.release(); e->dDirty.release(); e->dSur.release();
+  e->dStat.release(); e->dFlag.release(); e->dSur.release();
   e->Runtime::destroy();
   if (e->hS) (void)hipHostFree(e->hS);
   if (e->hFlagQ) (void)hipHostFree(e->hFlagQ);

@@ -35,10 +35,8 @@ struct wx_ctx : Mesh, Runtime {
   int32_t tLeafCap = 0;
   DevBuf<double> dStat;
   DevBuf<int8_t> dFlag;
-  DevBuf<MeshOp> dMeshOps;
   DevBuf<double> dScratch;
   std::vector<LeafRec> dirty;
-  DevBuf<LeafRec> dDirty;
   // pinned staging of the device scalars and the flags
   DevScalars *hS = nullptr, *hS_dev = nullptr;   // mapped: k_gather_flags writes it
   int8_t *hFlag = nullptr;
@@ -106,21 +104,24 @@ int upload_leaves(wx_ctx *c) {
   return stage_end(c);
 }
 
-// Second half: the ops of both fills, and the leaf records again when the
-// list set imposition flags in them.
+// Second half: the ops of both fills (fill I's only where they differ from
+// fill II's, which the fill then reads instead), and the leaf records again
+// when the list set imposition flags in them.
 int upload_ops(wx_ctx *c) {
   const size_t nleaf = c->leaves.size();
   const size_t bo = nleaf * 8 * sizeof(int2), bl = nleaf * sizeof(LeafRec);
-  const bool flags = c->p.otherb != 0;
-  int rc = stage_begin(c, 2 * al256(bo) + (flags ? al256(bl) : 0));
+  const bool flags = c->p.otherb != 0, two = !c->opsI_same;
+  int rc = stage_begin(c, (two ? 2 : 1) * al256(bo) + (flags ? al256(bl) : 0));
   if (rc) return rc;
-  char *h2 = stage_take(c, bo), *h1 = stage_take(c, bo);
+  char *h2 = stage_take(c, bo), *h1 = two ? stage_take(c, bo) : nullptr;
   char *hl = flags ? stage_take(c, bl) : nullptr;
-  if (!h2 || !h1 || (flags && !hl)) return fail(WX_EARG, "staging %zu bytes", 2 * bo + bl);
+  if (!h2 || (two && !h1) || (flags && !hl)) return fail(WX_EARG, "staging %zu bytes", 2 * bo + bl);
   memcpy(h2, c->opsII.data(), bo);
-  memcpy(h1, c->opsI.data(), bo);
   if ((rc = stage_push(c, c->tOpsII, h2, bo))) return rc;
-  if ((rc = stage_push(c, c->tOpsI, h1, bo))) return rc;
+  if (two) {
+    memcpy(h1, c->opsI.data(), bo);
+    if ((rc = stage_push(c, c->tOpsI, h1, bo))) return rc;
+  }
   if (flags) {
     memcpy(hl, c->leaves.data(), bl);
     if ((rc = stage_push(c, c->tLeaves, hl, bl))) return rc;
@@ -174,15 +175,15 @@ int update_stats(wx_ctx *c) {
     if (id >= 1 && id <= c->nbmax && c->lidx[id] >= 0) c->dirty.push_back(c->leaves[c->lidx[id]]);
   c->new_ids.clear();
   if (c->dirty.empty()) return WX_OK;
-  if ((rc = c->dDirty.ensure(c->dirty.size()))) return rc;
-  if ((rc = stage_begin(c, al256(c->dirty.size() * sizeof(LeafRec))))) return rc;
-  if ((rc = stage_copy(c, c->dDirty.ptr, c->dirty.data(), c->dirty.size() * sizeof(LeafRec)))) return rc;
-  if ((rc = stage_end(c))) return rc;
-  ProfScope ps(c, KID_STAT);
-  rc = launch_blockstat(c->dP.ptr, c->neq, c->dDirty.ptr, (int)c->dirty.size(), c->cur_u(), c->dStat.ptr,
-                        c->dFlag.ptr, c->stream);
+  const LeafRec *drec = (const LeafRec *)stage_inplace(c, c->dirty.data(), c->dirty.size() * sizeof(LeafRec));
+  if (!drec) return WX_ENOMEM;
+  {
+    ProfScope ps(c, KID_STAT);
+    rc = launch_blockstat(c->dP.ptr, c->neq, drec, (int)c->dirty.size(), c->cur_u(), c->dStat.ptr, c->dFlag.ptr,
+                          c->stream);
+  }
   if (rc) return fail(rc, "blockstat launch failed");
-  return WX_OK;
+  return stage_end(c);
 }
 
 int ensure_stats(wx_ctx *c) {
@@ -220,11 +221,12 @@ int run_stages_bf(wx_ctx *c) {
   if (!rc) {
     ProfScope ps(c, KID_FILLI);
     const FlagOut fo{c->tOff, c->hFlagQ_dev, c->hS_dev, 1};
+    const int2 *opsI = c->opsI_same ? c->tOpsII : c->tOpsI;
     if (c->use_sur)
-      rc = launch_fillI_cool_sur(c->dP.ptr, c->neq, c->queue(), g, 256, 0, c->tOpsI, c->tBinfo, c->cur_u(),
+      rc = launch_fillI_cool_sur(c->dP.ptr, c->neq, c->queue(), g, 256, 0, opsI, c->tBinfo, c->cur_u(),
                                  c->other_u(), c->dStat.ptr, c->dFlag.ptr, c->dS.ptr, c->dSur.ptr, fo, c->stream);
     else
-      rc = launch_fillI_cool(c->dP.ptr, c->neq, c->queue(), g, 256, 0, c->tOpsI, c->tBinfo, c->cur_u(),
+      rc = launch_fillI_cool(c->dP.ptr, c->neq, c->queue(), g, 256, 0, opsI, c->tBinfo, c->cur_u(),
                              c->other_u(), c->dStat.ptr, c->dFlag.ptr, c->dS.ptr, fo, c->stream);
   }
   if (rc) return fail(rc, "stage launch failed: %s", hipGetErrorString(hipGetLastError()));
@@ -250,17 +252,17 @@ int run_meshops(wx_ctx *c, std::vector<MeshOp> &ops, bool stats) {
       o.level = c->LP(o.kind == 0 ? o.s1 : o.a, 1);
       o.inst = 0;
     }
-  int rc = c->dMeshOps.ensure(ops.size());
-  if (rc) return rc;
-  if ((rc = stage_begin(c, al256(ops.size() * sizeof(MeshOp))))) return rc;
-  if ((rc = stage_copy(c, c->dMeshOps.ptr, ops.data(), ops.size() * sizeof(MeshOp)))) return rc;
-  if ((rc = stage_end(c))) return rc;
+  // the kernel reads the op list in place from the pinned staging
+  const MeshOp *dops = (const MeshOp *)stage_inplace(c, ops.data(), ops.size() * sizeof(MeshOp));
+  if (!dops) return WX_ENOMEM;
+  int rc;
   {
     ProfScope ps(c, KID_MESH);
-    rc = launch_meshops(c->dP.ptr, c->neq, c->dMeshOps.ptr, (int)ops.size(), c->cur_u(), stats ? c->dStat.ptr : nullptr,
+    rc = launch_meshops(c->dP.ptr, c->neq, dops, (int)ops.size(), c->cur_u(), stats ? c->dStat.ptr : nullptr,
                         stats ? c->dFlag.ptr : nullptr, c->stream);
   }
   if (rc) return fail(rc, "meshops launch failed");
+  if ((rc = stage_end(c))) return rc;
   if (stats) c->new_ids.clear();   // the new leaves' statistics are done
   return WX_OK;
 }
@@ -365,7 +367,7 @@ int wx_destroy(wx_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto &b : c->U) b.release();
   c->dP.release(); c->dS.release(); c->dTab.release(); c->dStat.release(); c->dFlag.release();
-  c->dMeshOps.release(); c->dScratch.release(); c->dDirty.release(); c->dSur.release();
+  c->dScratch.release(); c->dSur.release();
   c->Runtime::destroy();
   if (c->hS) (void)hipHostFree(c->hS);
   if (c->hFlag) (void)hipHostFree(c->hFlag);

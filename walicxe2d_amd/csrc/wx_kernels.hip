@@ -799,46 +799,47 @@ template <int NEQ>
 __global__ __launch_bounds__(256) void k_meshops(const wx_params *__restrict__ P, const MeshOp *__restrict__ ops,
                                                  double *__restrict__ U, double *__restrict__ stat,
                                                  int8_t *__restrict__ flag) {
-  const MeshOp op = ops[blockIdx.x];
+  // workgroup 4 * o + s: son s of a refine op o; a coarse op's dad is done
+  // by its first workgroup
+  const MeshOp op = ops[blockIdx.x >> 2];
+  const int part = blockIdx.x & 3;
   const int tid = threadIdx.x;
   __shared__ double d[NEQ * NC];   // coarse: the dad tile; then a tile of primitives for the statistics
   if (op.kind == 0) {
     const double *pb = U + (size_t)op.a * NEQ * NC;
-    const int sons[4] = {op.s1, op.s2, op.s3, op.s4};
-    for (int c = tid; c < 4 * NC; c += blockDim.x) {
-      const int s = c / NC, cc = c % NC;
+    const int s = part;
+    const int son = s == 0 ? op.s1 : s == 1 ? op.s2 : s == 2 ? op.s3 : op.s4;
+    double *sb = U + (size_t)son * NEQ * NC;
+    for (int cc = tid; cc < NC; cc += blockDim.x) {
       const int i = cc % NP - 1, j = cc / NP - 1;
       const int si = (i + 1) / 2 + ((s & 1) ? NX / 2 : 0);
       const int sj = (j + 1) / 2 + ((s & 2) ? NY / 2 : 0);
-      double *sb = U + (size_t)sons[s] * NEQ * NC;
 #pragma unroll
       for (int k = 0; k < NEQ; ++k) sb[k * NC + cc] = pb[k * NC + IDX(si, sj)];
     }
     if (!stat) return;
-    // statistics of each son from its cells 0..17, which are the parent's
+    // statistics of the son from its cells 0..17, which are the parent's
     // cells it was injected from (read back from the parent, not the son)
     const wx_params *__restrict__ p = P + op.inst;
-    for (int s = 0; s < 4; ++s) {
-      for (int c = tid; c < 18 * 18; c += blockDim.x) {
-        const int i = c % 18, j = c / 18;
-        const int si = (i + 1) / 2 + ((s & 1) ? NX / 2 : 0);
-        const int sj = (j + 1) / 2 + ((s & 2) ? NY / 2 : 0);
-        double uu[NEQ], q[NEQ], T;
+    for (int c = tid; c < 18 * 18; c += blockDim.x) {
+      const int i = c % 18, j = c / 18;
+      const int si = (i + 1) / 2 + ((s & 1) ? NX / 2 : 0);
+      const int sj = (j + 1) / 2 + ((s & 2) ? NY / 2 : 0);
+      double uu[NEQ], q[NEQ], T;
 #pragma unroll
-        for (int k = 0; k < NEQ; ++k) uu[k] = pb[k * NC + IDX(si, sj)];
-        uprim<NEQ>(p, q, uu, T);
+      for (int k = 0; k < NEQ; ++k) uu[k] = pb[k * NC + IDX(si, sj)];
+      uprim<NEQ>(p, q, uu, T);
 #pragma unroll
-        for (int k = 0; k < NEQ; ++k) d[k * NC + IDX(i, j)] = q[k];
-      }
-      __syncthreads();
-      LeafRec L{};
-      L.id = sons[s];
-      L.level = op.level;
-      L.inst = op.inst;
-      block_stats<NEQ>(p, d, L, stat, flag, FlagOut{nullptr, nullptr, nullptr, 0});
-      __syncthreads();   // d is refilled for the next son
+      for (int k = 0; k < NEQ; ++k) d[k * NC + IDX(i, j)] = q[k];
     }
+    __syncthreads();
+    LeafRec L{};
+    L.id = son;
+    L.level = op.level;
+    L.inst = op.inst;
+    block_stats<NEQ>(p, d, L, stat, flag, FlagOut{nullptr, nullptr, nullptr, 0});
   } else {
+    if (part != 0) return;
     const int sons[4] = {op.s1, op.s2, op.s3, op.s4};
     // start from the current dad tile (corners the rules below do not set)
     const double *dbg = U + (size_t)op.a * NEQ * NC;
@@ -1028,7 +1029,7 @@ int launch_gather_flags(QueueRef q, const int32_t *inst_off, int grid, const int
 int launch_meshops(const wx_params *P, int neq, const MeshOp *ops, int nops, double *U, double *stat, int8_t *flag,
                    hipStream_t stream) {
   if (nops <= 0) return WX_OK;
-  WX_DISPATCH(neq, hipLaunchKernelGGL(k_meshops<N>, dim3(nops), dim3(256), 0, stream, P, ops, U, stat, flag));
+  WX_DISPATCH(neq, hipLaunchKernelGGL(k_meshops<N>, dim3(4 * nops), dim3(256), 0, stream, P, ops, U, stat, flag));
   return hip_status();
 }
 

@@ -249,6 +249,7 @@ int compile_ops(Mesh *c) {
       }
     }
   }
+  c->opsI_same = late.empty();
   // every segment of fill II is written by the list (boundaryI may leave
   // segments to the carried values)
   for (size_t k = 0; k < nseg; ++k)

@@ -96,6 +96,23 @@ int stage_push(Runtime *r, void *dst, const char *host, size_t bytes) {
   return WX_OK;
 }
 
+const void *stage_dev(Runtime *r, const char *host) {
+  Runtime::Stage &st = r->stage[r->stage_cur];
+  if (host < st.buf || host >= st.buf + st.cap) return nullptr;
+  return st.dbuf + (host - st.buf);
+}
+
+const void *stage_inplace(Runtime *r, const void *src, size_t bytes) {
+  if (stage_begin(r, al256(bytes ? bytes : 1))) return nullptr;
+  char *h = stage_take(r, bytes ? bytes : 1);
+  if (!h) {
+    fail(WX_EARG, "staging %zu bytes", bytes);
+    return nullptr;
+  }
+  memcpy(h, src, bytes);
+  return stage_dev(r, h);
+}
+
 int stage_end(Runtime *r) {
   Runtime::Stage &st = r->stage[r->stage_cur];
   const size_t n = st.desc.size();

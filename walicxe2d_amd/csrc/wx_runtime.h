@@ -72,6 +72,7 @@ struct Mesh {
   std::vector<LeafRec> leaves;
   std::vector<BlockRec> binfo;
   std::vector<int2> opsII, opsI;
+  bool opsI_same = false;        // fill I's ops equal fill II's (no OUTFYB entries)
   std::vector<int> lidx;
   std::vector<int> segE2, segE1, ifirst, ilast, segLate;   // compile_ops scratch
   bool tables_valid = false;     // leaves / binfo / ops match the tree and the list
@@ -151,6 +152,13 @@ int stage_copy(Runtime *r, void *dst, const void *src, size_t bytes);
 // (nullptr if it does not fit); stage_push enqueues its copy to dst.
 char *stage_take(Runtime *r, size_t bytes);
 int stage_push(Runtime *r, void *dst, const char *host, size_t bytes);
+// Device alias of host memory taken from the open staging buffer, for a
+// kernel that reads it in place (launched before stage_end, which then
+// guards the buffer until that kernel has run).
+const void *stage_dev(Runtime *r, const char *host);
+// stage_begin + a copy of `bytes` from src into the staging buffer; returns
+// its device alias (nullptr on failure, with the error recorded).
+const void *stage_inplace(Runtime *r, const void *src, size_t bytes);
 // Launches the round's copies and records the buffer's event.
 int stage_end(Runtime *r);
 void prof_collect(Runtime *r);
