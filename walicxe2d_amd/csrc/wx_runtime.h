@@ -3,7 +3,8 @@
 //   Mesh     the host tree of one AMR mesh (the reference's lp/leaf/icoord,
 //            1-based block IDs), its ghost list and the gather tables
 //            compiled from it, and the admesh tree logic (wx_mesh.cpp);
-//   Runtime  one HIP stream with pinned double-buffered upload staging and
+//   Runtime  one HIP stream with a ring of pinned upload staging buffers
+//            (copied by one kernel per round, or read in place) and
 //            per-kernel event timing (wx_runtime.cpp).
 #pragma once
 #include <hip/hip_runtime.h>
