@@ -219,17 +219,17 @@ __device__ __forceinline__ int riemann(const wx_params *__restrict__ p, const do
 // Cooling.  src/coolingh.f90:41-76 rates, :88-165 ALOSS, :171-259 atomic.
 __device__ __forceinline__ double rate_alpha(double T) { return 2.55e-13 * pow(1.e4 / T, 0.79); }
 __device__ __forceinline__ double rate_colf(double T) { return 5.83e-11 * sqrt(T) * exp(-(157828. / T)); }
-// Recombination cooling rate, used only by ALOSS (rT = 1/T).
+// Recombination cooling rate, used only by ALOSS (rT = 1/T, sT = sqrt(T)).
 // a**(-0.33333) is taken as exp(-0.33333 log a) with the log the formula
-// needs anyway: a few ulps from the power function, at a fifth of its cost.
-// (The loss enters only the temperature relaxation, which does not amplify
-// it; the rates alpha and colf, which the stiff neutral-fraction update
-// amplifies ~2000-fold, keep the power and exponential functions and their
-// divisions.)
-__device__ __forceinline__ double rate_betah(double T, double rT) {
+// needs anyway (a few ulps from the power function, at a fifth of its
+// cost), and 1/sqrt(a) = sqrt(T) / sqrt(157890).  (The loss enters only the
+// temperature relaxation, which does not amplify it; the rates alpha and
+// colf, which the stiff neutral-fraction update amplifies ~2000-fold, keep
+// the power and exponential functions and their divisions.)
+__device__ __forceinline__ double rate_betah(double rT, double sT) {
   double a = 157890. * rT;
   double la = log(a);
-  return 1.133e-24 / sqrt(a) * (-0.0713 + 0.5 * la + 0.640 * exp(-0.33333 * la));
+  return 1.133e-24 * (sT * 0.0025166492284451375) * (-0.0713 + 0.5 * la + 0.640 * exp(-0.33333 * la));
 }
 
 __device__ __forceinline__ double aloss(double x1, double den, double dh0, double te0) {
@@ -254,7 +254,8 @@ __device__ __forceinline__ double aloss(double x1, double den, double dh0, doubl
     double FRAC = (TE - 55000.) / 17000.;
     OMEGA = (1. - FRAC) * OMEGAL + FRAC * OMEGAH;
   }
-  double QLA = 8.6287E-6 / (2. * sqrt(TE)) * OMEGA * exp(-(ENK * TM));
+  const double STE = sqrt(TE);
+  double QLA = 4.31435e-06 * (STE * TM) * OMEGA * exp(-(ENK * TM));   // 8.6287e-6 / (2 sqrt(TE))
   double ECOLL = DE * dh0 * QLA * EN;
   ECOLL = dmax(ECOLL, 0.);
   double CION = 5.834E-11 * sqrt(TE) * exp(-(1.579E5 * TM));
@@ -273,7 +274,7 @@ __device__ __forceinline__ double aloss(double x1, double den, double dh0, doubl
   // as the routine's product with a zero weight).
   double LOW = 0.0, HIGH = 0.0;
   if (TE < 54770.) {
-    double EREC = DE * DHP * (rate_betah(TE, TM));
+    double EREC = DE * DHP * (rate_betah(TM, STE));
     EREC = dmax(EREC, 0.);
     double T2 = TM * TM;
     // 10.**x of the routine as exp10 (within an ulp of the power function,
@@ -316,9 +317,9 @@ __device__ __forceinline__ void atomic_h(const wx_params *__restrict__ p, double
     y1 = dmax(y1, 0.0001);
     uu[4] = y1 * uu[0];
     double dh0 = uu[4];
-    double al = aloss(y0, dh, dh0, T) / (dh * dh);
+    // al = ALOSS / dh^2 and ce = 2 dh al / (3 k T) as one quotient (loss path)
     double tprime = 10.;
-    double ce = (2. * dh * al) / (3. * boltzm * T);
+    double ce = (2. * aloss(y0, dh, dh0, T)) / (3. * boltzm * T * dh);
     double t1 = tprime + (T - tprime) * exp(-(ce * dt));
     t1 = dmax(t1, 0.1 * T);
     t1 = dmin(t1, 10. * T);
