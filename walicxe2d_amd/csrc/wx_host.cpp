@@ -33,6 +33,7 @@ struct wx_ctx : Mesh, Runtime {
   int2 *tOpsII = nullptr, *tOpsI = nullptr;
   BlockRec *tBinfo = nullptr;
   int32_t tLeafCap = 0;
+  std::vector<BlockRec> tBinfoHost;   // what the arena's block records hold (empty: unknown)
   DevBuf<double> dStat;
   DevBuf<int8_t> dFlag;
   DevBuf<double> dScratch;
@@ -77,6 +78,7 @@ int arena_fit(wx_ctx *c) {
   c->tOpsI = (int2 *)((char *)c->tOpsII + al256(lcap * 8 * sizeof(int2)));
   c->tBinfo = (BlockRec *)((char *)c->tOpsI + al256(lcap * 8 * sizeof(int2)));
   c->tLeafCap = (int32_t)lcap;
+  c->tBinfoHost.clear();   // new arena: every block record goes up
   return WX_OK;
 }
 
@@ -87,20 +89,44 @@ int upload_leaves(wx_ctx *c) {
   int rc = arena_fit(c);
   if (rc) return rc;
   const size_t nleaf = c->leaves.size();
-  const size_t bl = nleaf * sizeof(LeafRec), bb = c->binfo.size() * sizeof(BlockRec);
-  if ((rc = stage_begin(c, al256(512 + bl) + al256(bb)))) return rc;
+  const size_t bl = nleaf * sizeof(LeafRec), nb = c->binfo.size();
+  // block records: only the runs that differ from what the arena holds (a
+  // mesh change alters the records of a few new blocks)
+  std::vector<BlockRec> &dev = c->tBinfoHost;
+  const BlockRec *bi = c->binfo.data();
+  std::vector<std::pair<size_t, size_t>> runs;   // [first, last) record indices
+  size_t nchanged = 0;
+  if (dev.size() != nb) {
+    runs.emplace_back(0, nb);
+    nchanged = nb;
+  } else {
+    for (size_t k = 0; k < nb;) {
+      if (memcmp(&bi[k], &dev[k], sizeof(BlockRec)) == 0) { ++k; continue; }
+      size_t e = k + 1;
+      while (e < nb && memcmp(&bi[e], &dev[e], sizeof(BlockRec)) != 0) ++e;
+      runs.emplace_back(k, e);
+      nchanged += e - k;
+      k = e;
+    }
+  }
+  if ((rc = stage_begin(c, al256(512 + bl) + runs.size() * 256 + al256(nchanged * sizeof(BlockRec))))) return rc;
   char *h = stage_take(c, 512 + bl);
-  char *hb = stage_take(c, bb);
-  if (!h || !hb) return fail(WX_EARG, "staging %zu bytes", 512 + bl + bb);
+  if (!h) return fail(WX_EARG, "staging %zu bytes", 512 + bl);
   const QueueHdr qh{(int32_t)nleaf, 1, {0, 0}};
   const int32_t off[2] = {0, (int32_t)nleaf};
   memset(h, 0, 512);
   memcpy(h, &qh, sizeof qh);
   memcpy(h + 256, off, sizeof off);
   memcpy(h + 512, c->leaves.data(), bl);
-  memcpy(hb, c->binfo.data(), bb);
   if ((rc = stage_push(c, c->dTab.ptr, h, 512 + bl))) return rc;
-  if ((rc = stage_push(c, c->tBinfo, hb, bb))) return rc;
+  for (const auto &r : runs) {
+    const size_t bytes = (r.second - r.first) * sizeof(BlockRec);
+    char *hb = stage_take(c, bytes);
+    if (!hb) return fail(WX_EARG, "staging %zu bytes", bytes);
+    memcpy(hb, bi + r.first, bytes);
+    if ((rc = stage_push(c, c->tBinfo + r.first, hb, bytes))) return rc;
+  }
+  dev.assign(bi, bi + nb);
   return stage_end(c);
 }
 
