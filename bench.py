@@ -316,8 +316,8 @@ def main():
         print(json.dumps(cpu_baseline_reference(P.CONFIGS[args.workload]())), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py: --gpus {args.gpus} is not supported: this build runs every leaf block on one "
-              "GPU (multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)
+        print(f"bench.py: --gpus {args.gpus} is not supported: the library keeps every leaf block on one "
+              "GPU (sharding across GPUs is not implemented); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     from walicxe2d_amd import params as P
