@@ -329,7 +329,9 @@ static int admesh_tree(Mesh *c, std::vector<uint8_t> &up, std::vector<uint8_t> &
   int nbmaxp = c->nbmax;
   const int nbmaxp1 = c->nbmax;
   // critneighup: closure of "refine my coarser neighbours", finest level first
-  std::vector<std::vector<int>> byl((size_t)nlevs + 2);
+  std::vector<std::vector<int>> &byl = c->admByLevel;   // scratch kept between calls
+  byl.resize((size_t)nlevs + 2);
+  for (auto &v : byl) v.clear();
   for (int nb : upl) byl[c->LP(nb, 1)].push_back(nb);
   for (int nl = nlevs - 1; nl >= 1; --nl)
     for (size_t k = 0; k < byl[nl].size(); ++k) {
@@ -367,7 +369,9 @@ static int admesh_tree(Mesh *c, std::vector<uint8_t> &up, std::vector<uint8_t> &
   // keep their flag) by level, each level in ascending ID order.  The
   // coarsening below changes neither levels, sibling numbers nor flags, so
   // the candidates of every level are known before it starts.
-  std::vector<std::vector<int>> cand((size_t)nlevs + 2);
+  std::vector<std::vector<int>> &cand = c->admCand;
+  cand.resize((size_t)nlevs + 2);
+  for (auto &v : cand) v.clear();
   {
     const int32_t *lp = c->lp.data(), *leaf = c->leaf.data();
     uint8_t *dnp = dn.data();
@@ -430,7 +434,9 @@ int refresh_tables(Mesh *c) {
 }
 
 int admesh_from_flags(Mesh *c, const int8_t *flag_of_leaf, std::vector<MeshOp> &ops) {
-  std::vector<int> upl, dnl;
+  std::vector<int> &upl = c->admUp, &dnl = c->admDown;   // scratch kept between calls
+  upl.clear();
+  dnl.clear();
   const size_t n0 = ops.size();
   for (size_t l = 0; l < c->leaves.size(); ++l) {   // markref, src/markref.f90:37-44
     const int id = c->leaves[l].id;

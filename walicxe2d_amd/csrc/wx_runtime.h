@@ -82,8 +82,10 @@ struct Mesh {
   // after a mesh change only the new leaves need them recomputed
   bool stats_all_stale = true;
   std::vector<int> new_ids;
-  // admesh scratch flags per block ID (all zero between calls)
+  // admesh scratch flags per block ID (all zero between calls), and its lists
   std::vector<uint8_t> up, dn;
+  std::vector<int> admUp, admDown;
+  std::vector<std::vector<int>> admByLevel, admCand;
 
   int LP(int nb, int k) const { return lp[(size_t)nb * 8 + k - 1]; }
   void SETLP(int nb, int k, int v) { lp[(size_t)nb * 8 + k - 1] = v; }
