@@ -72,8 +72,10 @@ struct DevScalars {
   double dt;
   double dtp_min;
   double dtcool_min;
+  double vjet, vjet_time;   // jet source speed at vjet_time, valid when vjet_ok (set by k_dtreduce)
   int32_t err;
-  int32_t pad[7];
+  int32_t vjet_ok;
+  int32_t pad[2];
 };
 
 // Mesh data ops for admesh (src/refine.f90, src/coarse.f90).

@@ -293,7 +293,7 @@ __global__ __launch_bounds__(BS) void k_stageB(const wx_params *__restrict__ P, 
   const int b = L.id;
   const double time = S->time;
   const bool imp = (L.flags & 1) != 0;
-  const double vjet = p->otherb ? jet_velocity(p, time) : 0.0;
+  const double vjet = p->otherb ? jet_velocity_at(p, S, time) : 0.0;
   const double *qb = Q + (size_t)b * NEQ * NC;
   double *ub = U + (size_t)b * NEQ * NC;
   const int tid = threadIdx.x;
@@ -606,7 +606,7 @@ __global__ __launch_bounds__(BS, (SUR ? 2 : 1024 / BS)) void k_fillI_cool(const 
   const int tid = threadIdx.x;
   const double time = S->time;
   const bool imp = (L.flags & 1) != 0;
-  const double vjet = p->otherb ? jet_velocity(p, time) : 0.0;
+  const double vjet = p->otherb ? jet_velocity_at(p, S, time) : 0.0;
   const double *ib = Uin + (size_t)b * NEQ * NC;
   // carried tile (interior U^{n+1}, ring of U^{n+1/2}); impose where listed
   for (int c = tid; c < NC; c += blockDim.x) {
@@ -745,6 +745,12 @@ __global__ __launch_bounds__(1024) void k_dtreduce(const wx_params *__restrict__
     double dt = dmin(p->cfl * a, 0.4 * c * p->Psc / p->tsc);
     S->dt = dt;
     if (advance) S->time = S->time + dt;
+    if (p->otherb) {   // the jet source speed of the step, once instead of per wave of the stages
+      const double t = S->time;
+      S->vjet = jet_velocity(p, t);
+      S->vjet_time = t;
+      S->vjet_ok = 1;
+    }
   }
 }
 

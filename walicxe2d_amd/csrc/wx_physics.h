@@ -384,4 +384,12 @@ __device__ __forceinline__ double jet_velocity(const wx_params *__restrict__ p, 
   return p->jet[6] + p->jet[7] * sin(2. * pi * time / p->jet[8]);
 }
 
+// The source speed at `time`: the value the dt reduction stored for that
+// time (the same function of the same time), else computed here.
+__device__ __forceinline__ double jet_velocity_at(const wx_params *__restrict__ p, const DevScalars *S,
+                                                  double time) {
+  if (S->vjet_ok && S->vjet_time == time) return S->vjet;
+  return jet_velocity(p, time);
+}
+
 }  // namespace wx
