@@ -194,6 +194,44 @@ def test_reference_interface_path():
     sim.close()
 
 
+def test_jet_through_timestep_tstep():
+    """The jet driven as the Fortran host drives it (wx_timestep, then
+    wx_tstep with the host's time and dt), mixed with wx_step: the stages
+    take the jet speed that the dt reduction stored only for the time it
+    belongs to, and compute it themselves for a time the host set.  Per step
+    against the oracle: density and momenta within 1e-12, energy and n_HI
+    within 1e-11."""
+    cfg = P.config_c3(nblocks=6000, nbrootx=16, nbrooty=4)
+    O = Oracle()
+    pp, m = oracle_mesh(O, cfg, 6000)
+    for _ in range(3):
+        O.admesh(pp, m)
+    sim = Simulation(cfg, nblocks=6000)
+    sim.upload(m.u, m.lp, m.leaf, m.icoord, m.nbmax, m.nbleafs)
+    t = 0.0
+    for s in range(8):
+        dt_o = O.timestep(pp, m)
+        t += dt_o
+        assert O.tstep(pp, m, t, dt_o) == 0
+        if s % 2 == 0:               # host-driven: timestep, then tstep(t, dt)
+            dt_g = sim.timestep()
+            assert abs(dt_g - dt_o) <= TOL * dt_o, (s, dt_g, dt_o)
+            sim.tstep(t, dt_o)
+        else:                        # device-driven: dt reduction, time += dt, stages
+            tg, dtg, _ = sim.step(1, admesh=False)
+            assert abs(dtg - dt_o) <= TOL * dt_o and abs(tg - t) <= TOL * t, (s, tg, t)
+        lv = m.leaves()
+        ug = sim.download()
+        e, where = rel_err(ug[lv], m.u[lv], MASK, nvar=3)
+        e5, where5 = rel_err(ug[lv], m.u[lv], MASK)
+        assert e <= TOL, f"step {s}: density/momentum rel err {e:.3e} at {where}"
+        assert e5 <= TOL_HI, f"step {s}: rel err {e5:.3e} at {where5}"
+        # resynchronise the device exactly onto the oracle (time included)
+        sim.upload(m.u, m.lp, m.leaf, m.icoord, m.nbmax, m.nbleafs)
+        sim.set_time(t)
+    sim.close()
+
+
 def test_fortran_layout_roundtrip():
     """wx_upload / wx_download in the reference's block-fastest layout
     u(nbmin:nbmaxtot, neq, -1:18, -1:18) (src/initmain.f90:76-78)."""
