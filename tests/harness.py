@@ -44,7 +44,7 @@ def cell_scale(b: np.ndarray, gamma: float = 5. / 3.) -> np.ndarray:
     component can cross zero while the cell's momentum scale does not), and
     the total energy for energy."""
     rho = np.maximum(np.abs(b[:, 0]), 1e-300)
-    v2 = (b[:, 1] ** 2 + b[:, 2] ** 2) / rho ** 2
+    v2 = (b[:, 1] / rho) ** 2 + (b[:, 2] / rho) ** 2     # rho**2 would underflow in unset cells
     eint = np.maximum(b[:, 3] - 0.5 * rho * v2, 1e-300)
     cs = np.sqrt(np.maximum(gamma * (gamma - 1.) * eint / rho, 0.0))
     sc = np.empty_like(b)
