@@ -173,6 +173,21 @@ int wx_shape(wx_ctx *ctx, int32_t *neq, int32_t *nblocks, int32_t *nlevs);
 int wx_write_bin(wx_ctx *ctx, const char *pointers_path, const char *blocks_path, int32_t logical_true);
 int wx_read_bin(wx_ctx *ctx, int32_t nfiles, const char *const *pointers_paths, const char *const *blocks_paths);
 
+/* Block placement plan of the MPI load balancer (src/loadbalance.f90:5-105)
+ * for a mesh held by one rank (IDs 1..nbmax): hilbert(nb) = the Hilbert key
+ * of the block's corner at the finest level over nlevs+1 bits per axis
+ * (hilbert2d, :251-306; higher bits ignored as there), and idnew(nb) = the
+ * block's new ID when the blocks, in key order (indexx, :320-400, with its
+ * order among equal keys), are split into nranks runs: rank r gets IDs
+ * r*nblocks + 1, 2, ... and closes after the block that brings its leaf count
+ * to nbleafs/nranks.  lp(ld,8), leaf(ld), icoord(ld,2) column-major as in
+ * wx_upload_tree; hilbert and idnew have nbmax entries.  Host only (no
+ * device work, no context).  WX_EBLOCKS where the reference stops (an ID
+ * reaching its rank's capacity).  The reference's remapping of lp, its data
+ * exchange and its MPI collectives are not part of the plan. */
+int wx_lb_plan(const int32_t *lp, const int32_t *leaf, const int32_t *icoord, int32_t ld, int32_t nbmax,
+               int32_t nlevs, int32_t nblocks, int32_t nranks, int32_t *hilbert, int32_t *idnew);
+
 /*
  * Batched ensemble: n independent AMR meshes ("instances"), each with its own
  * parameters (p[0..n-1], same neq), tree, time and timestep, advanced

@@ -1121,3 +1121,102 @@ int wxo_run(const wxo_params *p, wxo_mesh *m, int nsteps, double *time, double *
   }
   return err;
 }
+
+/* ---------------------------------------------------------------------
+ * Load-balance plan (src/loadbalance.f90).  Arrays here are C-indexed by
+ * block ID (slot 0 unused): lp[nb*8 + k-1] = lp(nb,k), icoord[nb*2 + d].
+ * ------------------------------------------------------------------- */
+
+/* src/loadbalance.f90:251-306 (hilbert2d): two key bits per level, most
+ * significant level first; the pair (x bit, y bit) as a digit 2*x+y walks a
+ * 4-state table (state_diagram(digit, 0|1, state) = next state | key digit).
+ * Bits of x and y at or above `bits` do not enter the key. */
+static const int8_t lb_next_state[4][4] = {{1, 0, 2, 0}, {0, 3, 1, 1}, {2, 2, 0, 3}, {3, 1, 3, 2}};
+static const int8_t lb_key_digit[4][4]  = {{0, 1, 3, 2}, {0, 3, 1, 2}, {2, 1, 3, 0}, {2, 3, 1, 0}};
+
+int32_t wxo_hilbert2d(int32_t x, int32_t y, int bits) {
+  int state = 0;
+  uint32_t key = 0;
+  for (int b = bits - 1; b >= 0; --b) {
+    int d = (((x >> b) & 1) << 1) | ((y >> b) & 1);
+    key |= (uint32_t)lb_key_digit[state][d] << (2 * b);
+    state = lb_next_state[state][d];
+  }
+  return (int32_t)key;
+}
+
+/* src/loadbalance.f90:320-400 (indexx): indx[0..n-1] (1-based values) such
+ * that arr[indx[k]-1] ascends; median-of-three quicksort that hands ranges
+ * of fewer than 6 elements (ir - l < 5) to straight insertion, larger part
+ * pushed.  The order among equal keys is this algorithm's and part of the
+ * result.  Work indices below are 1-based as in the source. */
+void wxo_indexx(int n, const int32_t *arr, int32_t *indx) {
+  int32_t *ix = indx - 1;                       /* ix[1..n] */
+  const int32_t *a = arr - 1;                   /* a[1..n]  */
+  int stack[64], sp = 0, l = 1, ir = n;
+  for (int k = 1; k <= n; ++k) ix[k] = k;
+#define LB_SWAP(p, q) do { int32_t t_ = ix[p]; ix[p] = ix[q]; ix[q] = t_; } while (0)
+  for (;;) {
+    if (ir - l < 5) {
+      for (int j = l + 1; j <= ir; ++j) {
+        int32_t v = ix[j], av = a[v];
+        int i = j - 1;
+        while (i >= l && a[ix[i]] > av) { ix[i + 1] = ix[i]; --i; }
+        ix[i + 1] = v;
+      }
+      if (sp == 0) return;
+      ir = stack[--sp];
+      l = stack[--sp];
+    } else {
+      int mid = (l + ir) / 2;
+      LB_SWAP(mid, l + 1);
+      if (a[ix[l]] > a[ix[ir]]) LB_SWAP(l, ir);
+      if (a[ix[l + 1]] > a[ix[ir]]) LB_SWAP(l + 1, ir);
+      if (a[ix[l]] > a[ix[l + 1]]) LB_SWAP(l, l + 1);
+      int i = l + 1, j = ir;
+      int32_t pv = ix[l + 1], ap = a[pv];
+      for (;;) {
+        do ++i; while (a[ix[i]] < ap);
+        do --j; while (a[ix[j]] > ap);
+        if (j < i) break;
+        LB_SWAP(i, j);
+      }
+      ix[l + 1] = ix[j];
+      ix[j] = pv;
+      if (ir - i + 1 >= j - l) { stack[sp++] = i; stack[sp++] = ir; ir = j - 1; }
+      else                     { stack[sp++] = l; stack[sp++] = j - 1; l = i; }
+    }
+  }
+#undef LB_SWAP
+}
+
+/* src/loadbalance.f90:24-33 (block coordinates at the finest level and their
+ * keys over nlevs+1 bits) and :73-105 (new IDs on the master) for blocks
+ * 1..nbmax held by one rank (idold(j) = j): blocks in key order fill rank
+ * 0, 1, ... with IDs r*nblocks + 1, 2, ...; a rank closes after the block
+ * that brings its leaf count to nbleafs/nranks (the last rank takes the
+ * rest).  key[nb], idnew[nb] for nb = 1..nbmax.  Returns WXO_ERR_BLOCKS
+ * where the source stops (an ID reaching the rank's capacity). */
+int wxo_lb_plan(const int32_t *lp, const int32_t *leaf, const int32_t *icoord, int nbmax, int nlevs,
+                int nblocks, int nranks, int32_t *key, int32_t *idnew) {
+  if (nbmax < 1 || nranks < 1 || nlevs < 1 || nlevs > 14) return WXO_ERR_ARG;
+  int nleaf = 0;
+  for (int nb = 1; nb <= nbmax; ++nb) {
+    int sh = nlevs - lp[nb * 8];
+    int32_t xh = icoord[nb * 2] * (1 << sh) / NX, yh = icoord[nb * 2 + 1] * (1 << sh) / NY;
+    key[nb] = wxo_hilbert2d(xh, yh, nlevs + 1);
+    nleaf += leaf[nb] != 0;
+  }
+  int32_t *order = (int32_t *)malloc(sizeof(int32_t) * (size_t)nbmax);
+  wxo_indexx(nbmax, key + 1, order);
+  int per = nleaf / nranks, r = 1, cnt = 1, base = 0, err = WXO_OK;
+  for (int k = 1; k <= nbmax; ++k) {
+    int nb = order[k - 1];
+    idnew[nb] = (r - 1) * nblocks + k - base;
+    if (leaf[nb]) ++cnt;
+    if (cnt > per && r < nranks) { ++r; cnt = 1; base = k; }
+    if (idnew[nb] / r == nblocks) { err = WXO_ERR_BLOCKS; break; }
+  }
+  free(order);
+  return err;
+}

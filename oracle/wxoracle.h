@@ -108,6 +108,12 @@ int    wxo_admesh_flags(const wxo_params *p, wxo_mesh *m, int32_t *irefup, int32
 void   wxo_basegrid(const wxo_params *p, wxo_mesh *m);
 int    wxo_run(const wxo_params *p, wxo_mesh *m, int nsteps, double *time, double *dtlast,
                int do_admesh, double *zones);
+/* Load-balance plan (src/loadbalance.f90): Hilbert keys, the index sort and
+ * the new block IDs for nranks ranks of a mesh held on one rank. */
+int32_t wxo_hilbert2d(int32_t x, int32_t y, int bits);
+void   wxo_indexx(int n, const int32_t *arr, int32_t *indx);
+int    wxo_lb_plan(const int32_t *lp, const int32_t *leaf, const int32_t *icoord, int nbmax, int nlevs,
+                   int nblocks, int nranks, int32_t *key, int32_t *idnew);
 #ifdef __cplusplus
 }
 #endif

@@ -121,6 +121,10 @@ class Oracle:
         L.wxo_admesh.argtypes = [P, M]
         L.wxo_basegrid.argtypes = [P, M]
         L.wxo_run.argtypes = [P, M, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp]
+        L.wxo_hilbert2d.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int]
+        L.wxo_hilbert2d.restype = ctypes.c_int32
+        L.wxo_indexx.argtypes = [ctypes.c_int, _ip, _ip]
+        L.wxo_lb_plan.argtypes = [_ip, _ip, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip]
 
     @staticmethod
     def _p(params):
@@ -223,6 +227,30 @@ class Oracle:
 
     def admesh(self, params, m):
         return self.L.wxo_admesh(self._p(params), ctypes.byref(m.s))
+
+    def hilbert2d(self, x, y, bits):
+        """src/loadbalance.f90:251-306: the Hilbert key of (x, y) over `bits` bits."""
+        return int(self.L.wxo_hilbert2d(int(x), int(y), int(bits)))
+
+    def indexx(self, arr):
+        """src/loadbalance.f90:320-400: 1-based index table of arr in ascending order."""
+        a = np.ascontiguousarray(arr, dtype=np.int32)
+        ix = np.zeros(len(a), dtype=np.int32)
+        self.L.wxo_indexx(len(a), iptr(a), iptr(ix))
+        return ix
+
+    def lb_plan(self, lp, leaf, icoord, nbmax, nlevs, nblocks, nranks):
+        """src/loadbalance.f90:24-33,73-105 for blocks 1..nbmax held on one
+        rank: (err, key[nbmax+1], idnew[nbmax+1]), slot 0 unused.  lp
+        [>nbmax][8], leaf [>nbmax], icoord [>nbmax][2], indexed by block ID."""
+        lp = np.ascontiguousarray(lp[:nbmax + 1], dtype=np.int32)
+        leaf = np.ascontiguousarray(leaf[:nbmax + 1], dtype=np.int32)
+        ic = np.ascontiguousarray(icoord[:nbmax + 1], dtype=np.int32)
+        key = np.zeros(nbmax + 1, dtype=np.int32)
+        idnew = np.zeros(nbmax + 1, dtype=np.int32)
+        err = self.L.wxo_lb_plan(iptr(lp), iptr(leaf), iptr(ic), nbmax, nlevs, nblocks, nranks,
+                                 iptr(key), iptr(idnew))
+        return err, key, idnew
 
     def basegrid(self, params, m):
         self.L.wxo_basegrid(self._p(params), ctypes.byref(m.s))

@@ -31,7 +31,8 @@ DEVICE = [
 # device translation units
 KERNEL_SOURCES = ["wx_kernels.hip", "wx_surrogate.hip"]
 # host-side translation units
-HOST_SOURCES = ["wx_mesh.cpp", "wx_runtime.cpp", "wx_host.cpp", "wx_ensemble.cpp", "wx_surrogate.cpp", "wx_bin.cpp"]
+HOST_SOURCES = ["wx_mesh.cpp", "wx_runtime.cpp", "wx_host.cpp", "wx_ensemble.cpp", "wx_surrogate.cpp", "wx_bin.cpp",
+                "wx_loadbalance.cpp"]
 
 
 def _run(cmd):

@@ -33,7 +33,7 @@ EXPORTS = [
     "wx_ens_profile", "wx_ens_info", "wx_ens_create_groups", "wx_ens_groups",
     "wx_shape", "wx_write_bin", "wx_read_bin", "wx_ens_shape", "wx_ens_write_bin",
     "wx_sur_pairs", "wx_sur_features", "wx_sur_infer", "wx_sur_grad", "wx_sur_train", "wx_set_surrogate",
-    "wx_ens_set_surrogate", "wx_sur_counters", "wx_ens_sur_counters",
+    "wx_ens_set_surrogate", "wx_sur_counters", "wx_ens_sur_counters", "wx_lb_plan",
 ]
 
 _fp = ctypes.POINTER(ctypes.c_float)
@@ -83,6 +83,7 @@ def load_library(path: str = LIBPATH) -> ctypes.CDLL:
     L.wx_info.argtypes = [P, _ip, _ip, _ip]
     L.wx_last_error.argtypes = [ctypes.c_char_p, ctypes.c_int32]
     L.wx_stream.argtypes = [P]
+    L.wx_lb_plan.argtypes = [_ip, _ip, _ip] + [ctypes.c_int32] * 5 + [_ip, _ip]
     L.wx_stream.restype = P
     L.wx_profile.argtypes = [P, ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]
     L.wx_ens_create.argtypes = [ctypes.POINTER(WxParams), ctypes.c_int32, ctypes.POINTER(P)]
@@ -147,6 +148,20 @@ def tree_to_fortran(lp: np.ndarray, leaf: np.ndarray, icoord: np.ndarray, nbmax:
     fleaf = np.ascontiguousarray(leaf[1:ld + 1], dtype=np.int32)
     fic = np.ascontiguousarray(icoord[1:ld + 1].T, dtype=np.int32)
     return flp, fleaf, fic, ld
+
+
+def lb_plan(lp: np.ndarray, leaf: np.ndarray, icoord: np.ndarray, nbmax: int, nlevs: int, nblocks: int,
+            nranks: int):
+    """wx_lb_plan on row-per-ID tree arrays (index 0 unused, as tree()
+    returns them): (hilbert[nbmax+1], idnew[nbmax+1]), slot 0 unused."""
+    L = load_library()
+    flp, fleaf, fic, ld = tree_to_fortran(lp, leaf, icoord, nbmax)
+    key = np.zeros(nbmax + 1, dtype=np.int32)
+    idnew = np.zeros(nbmax + 1, dtype=np.int32)
+    _check(L.wx_lb_plan(flp.ctypes.data_as(_ip), fleaf.ctypes.data_as(_ip), fic.ctypes.data_as(_ip), ld,
+                        nbmax, nlevs, nblocks, nranks, key[1:].ctypes.data_as(_ip),
+                        idnew[1:].ctypes.data_as(_ip)), "wx_lb_plan")
+    return key, idnew
 
 
 class Simulation:

@@ -70,6 +70,15 @@ module wx_gpu
        character(kind=c_char), intent(in) :: pointers(*), blocks(*)
        integer(c_int32_t), value :: logical_true
      end function wx_write_bin
+     ! Hilbert placement plan of src/loadbalance.f90:5-105 (host only): keys
+     ! and new block IDs of blocks 1..nbmax for nranks ranks
+     integer(c_int) function wx_lb_plan(lp, leaf, icoord, ld, nbmax, nlevs, nblocks, nranks, hilbert, idnew) &
+          bind(C, name="wx_lb_plan")
+       import :: c_int, c_int32_t
+       integer(c_int32_t), intent(in) :: lp(*), leaf(*), icoord(*)
+       integer(c_int32_t), value :: ld, nbmax, nlevs, nblocks, nranks
+       integer(c_int32_t), intent(out) :: hilbert(*), idnew(*)
+     end function wx_lb_plan
      integer(c_int) function wx_set_bounds(ctx, ib, ld, n) bind(C, name="wx_set_bounds")
        import :: c_int, c_ptr, c_int32_t
        type(c_ptr), value :: ctx
