@@ -92,6 +92,25 @@ def test_oracle_matches_reference_routines():
     assert np.array_equal(O.indexx(a), ref)
 
 
+def test_oracle_matches_reference_golden_vectors():
+    """tests/golden/lb_vectors.npz: outputs of the reference's own hilbert2d
+    and indexx on seeded inputs (tests/golden/make_lb_golden.py)."""
+    O = Oracle()
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "lb_vectors.npz"))
+    nh = ni = 0
+    for name in g.files:
+        if name.startswith("hilbert_") and name.endswith("_key"):
+            bits = int(name.split("_")[1])
+            x, y = g[f"hilbert_{bits}_x"], g[f"hilbert_{bits}_y"]
+            assert [O.hilbert2d(a, b, bits) for a, b in zip(x, y)] == list(g[name]), bits
+            nh += 1
+        elif name.startswith("indexx_") and name.endswith("_idx"):
+            a = g[name[:-4] + "_arr"]
+            assert np.array_equal(O.indexx(a), g[name]), name
+            ni += 1
+    assert nh == 5 and ni == 21
+
+
 def _jet_mesh(steps=30):
     """16x4 roots, 4 levels: the finest block index reaches 127, above the
     keys' 5 bits, so keys alias as in the reference and many tie (21 keys
