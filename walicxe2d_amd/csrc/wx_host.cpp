@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -497,6 +498,22 @@ int wx_locatebounds(wx_ctx *c, int32_t *ninternal) {
   return WX_OK;
 }
 
+// Jet source speed at a time the host sets (JET/jet.f90:55), computed here
+// with the host C library's sin -- the function the reference calls -- and
+// handed to the stages with that time.  (When the device advances the time
+// itself, the dt reduction computes the speed.)  Caller has synchronised the
+// stream, so the pinned scalars are free.
+static int stage_jet_speed(wx_ctx *c, double time) {
+  if (!c->p.otherb) return WX_OK;
+  const double pi = 3.141592653589793;  // acos(-1.)
+  c->hS->vjet = c->p.jet[6] + c->p.jet[7] * std::sin(2. * pi * time / c->p.jet[8]);
+  c->hS->vjet_time = time;
+  c->hS->vjet_ok = 1;
+  HIPCHK(hipMemcpyAsync(&c->dS.ptr->vjet, &c->hS->vjet, 2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(&c->dS.ptr->vjet_ok, &c->hS->vjet_ok, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  return WX_OK;
+}
+
 int wx_timestep(wx_ctx *c, double *dt) {
   if (!c || !dt) return fail(WX_EARG, "null argument");
   int rc = ensure_stats(c);
@@ -518,6 +535,8 @@ int wx_tstep(wx_ctx *c, double time, double dt) {
   c->hS->time = time;
   c->hS->dt = dt;
   HIPCHK(hipMemcpyAsync(c->dS.ptr, c->hS, 2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  rc = stage_jet_speed(c, time);
+  if (rc) return rc;
   return run_stages(c);
 }
 
@@ -580,6 +599,8 @@ int wx_set_time(wx_ctx *c, double time) {
   HIPCHK(hipStreamSynchronize(c->stream));
   c->hS->time = time;
   HIPCHK(hipMemcpyAsync(&c->dS.ptr->time, &c->hS->time, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  int rc = stage_jet_speed(c, time);
+  if (rc) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   return WX_OK;
 }

@@ -7,6 +7,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include "wx_device.h"
+#include "wx_libm.h"
 
 namespace wx {
 
@@ -217,19 +218,14 @@ __device__ __forceinline__ int riemann(const wx_params *__restrict__ p, const do
 
 // ---------------------------------------------------------------------
 // Cooling.  src/coolingh.f90:41-76 rates, :88-165 ALOSS, :171-259 atomic.
-__device__ __forceinline__ double rate_alpha(double T) { return 2.55e-13 * pow(1.e4 / T, 0.79); }
-__device__ __forceinline__ double rate_colf(double T) { return 5.83e-11 * sqrt(T) * exp(-(157828. / T)); }
-// Recombination cooling rate, used only by ALOSS (rT = 1/T, sT = sqrt(T)).
-// a**(-0.33333) is taken as exp(-0.33333 log a) with the log the formula
-// needs anyway (a few ulps from the power function, at a fifth of its
-// cost), and 1/sqrt(a) = sqrt(T) / sqrt(157890).  (The loss enters only the
-// temperature relaxation, which does not amplify it; the rates alpha and
-// colf, which the stiff neutral-fraction update amplifies ~2000-fold, keep
-// the power and exponential functions and their divisions.)
-__device__ __forceinline__ double rate_betah(double rT, double sT) {
-  double a = 157890. * rT;
-  double la = log(a);
-  return 1.133e-24 * (sT * 0.0025166492284451375) * (-0.0713 + 0.5 * la + 0.640 * exp(-0.33333 * la));
+// Every operation is the routine's, in its order; exp and ** are the host C
+// library's functions (wx_libm.h: equal to them bit for bit), log is
+// correctly rounded.
+__device__ __forceinline__ double rate_alpha(double T) { return 2.55e-13 * libm::wpow(1.e4 / T, 0.79); }
+__device__ __forceinline__ double rate_colf(double T) { return 5.83e-11 * sqrt(T) * libm::wexp(-(157828. / T)); }
+__device__ __forceinline__ double rate_betah(double T) {
+  double a = 157890. / T;
+  return 1.133e-24 / sqrt(a) * (-0.0713 + 0.5 * libm::wlog(a) + 0.640 * libm::wpow(a, -0.33333));
 }
 
 __device__ __forceinline__ double aloss(double x1, double den, double dh0, double te0) {
@@ -238,8 +234,6 @@ __device__ __forceinline__ double aloss(double x1, double den, double dh0, doubl
   const double D0 = 0.5856, D1 = 1.55083e-5, D2 = -9.669e-12, D3 = 5.716e-19;
   const double ENK = 118409., EN = 1.634E-11;
   double TE = dmax(te0, 1000.);
-  // x / TE as x * (1 / TE) in the loss terms (within an ulp)
-  const double TM = 1. / TE;
   double DH = den;
   double DHP = (1. - x1) * DH;
   double DE = DHP + 1.E-4 * DH;
@@ -254,17 +248,16 @@ __device__ __forceinline__ double aloss(double x1, double den, double dh0, doubl
     double FRAC = (TE - 55000.) / 17000.;
     OMEGA = (1. - FRAC) * OMEGAL + FRAC * OMEGAH;
   }
-  const double STE = sqrt(TE);
-  double QLA = 4.31435e-06 * (STE * TM) * OMEGA * exp(-(ENK * TM));   // 8.6287e-6 / (2 sqrt(TE))
+  double QLA = 8.6287E-6 / (2. * sqrt(TE)) * OMEGA * libm::wexp(-(ENK / TE));
   double ECOLL = DE * dh0 * QLA * EN;
   ECOLL = dmax(ECOLL, 0.);
-  double CION = 5.834E-11 * sqrt(TE) * exp(-(1.579E5 * TM));
+  double CION = 5.834E-11 * sqrt(TE) * libm::wexp(-(1.579E5 / TE));
   double EION = DE * dh0 * CION * XION;
   double FR = 0.0;
   if (TE <= 44770.) FR = 0.;
   if (TE >= 54770.) FR = 1.;
   if (TE > 44770. && TE < 54770.) {
-    double EX2 = exp(-(2. * (TE - 49770.) / 500.));
+    double EX2 = libm::wexp(-(2. * (TE - 49770.) / 500.));
     double TANH = (1. - EX2) / (1. + EX2);
     FR = 0.5 * (1. + TANH);
   }
@@ -274,23 +267,20 @@ __device__ __forceinline__ double aloss(double x1, double den, double dh0, doubl
   // as the routine's product with a zero weight).
   double LOW = 0.0, HIGH = 0.0;
   if (TE < 54770.) {
-    double EREC = DE * DHP * (rate_betah(TM, STE));
+    double EREC = DE * DHP * (rate_betah(TE));
     EREC = dmax(EREC, 0.);
+    double TM = 1. / TE;
     double T2 = TM * TM;
-    // 10.**x of the routine as exp10 (within an ulp of the power function,
-    // at a fraction of its cost)
-    double EOI = DE * DOI * exp10(1381465 * T2 - 12328.69 * TM - 19.82621);
-    double EOII = DE * DOII * exp10(-(2061075. * T2) - 14596.24 * TM - 19.01402);
+    double EOI = DE * DOI * libm::wpow(10., 1381465 * T2 - 12328.69 * TM - 19.82621);
+    double EOII = DE * DOII * libm::wpow(10., -(2061075. * T2) - 14596.24 * TM - 19.01402);
     EOI = dmax(EOI, 0.);
     EOII = dmax(EOII, 0.);
     LOW = EREC + 7.033 * (EOI + EOII);
   }
   if (TE > 44770.) {
-    // TE**0.5 as sqrt; TE**0.63 and (TE*1e-5)**1.63 as exp(y log x) (a few
-    // ulps, in the loss only, as for betah)
-    double BETAF = 1.3 * 1.42E-27 * sqrt(TE);
+    double BETAF = 1.3 * 1.42E-27 * libm::wpow(TE, 0.5);
     double HIICOOL = DE * DHP * BETAF;
-    HIGH = (1.0455E-18 / exp(0.63 * log(TE))) * (1. - exp(-exp(1.63 * log(TE * 1.E-5)))) * DE * den + HIICOOL;
+    HIGH = (1.0455E-18 / libm::wpow(TE, 0.63)) * (1. - libm::wexp(-libm::wpow(TE * 1.E-5, 1.63))) * DE * den + HIICOOL;
   }
   return ECOLL + EION + LOW * (1. - FR) + HIGH * FR;
 }
@@ -311,16 +301,16 @@ __device__ __forceinline__ void atomic_h(const wx_params *__restrict__ p, double
     double c = (1. + xi) * rec;
     double d = sqrt(b * b - 4. * a * c);
     double g0 = (2. * a * y0 + b + d) / (2. * a * y0 + b - d);
-    double e = exp(-(d * dh * dt));
+    double e = libm::wexp(-(d * dh * dt));
     double y1 = (-b - d * (1. + g0 * e) / (1. - g0 * e)) / (2. * a);
     y1 = dmin(y1, 1.);
     y1 = dmax(y1, 0.0001);
     uu[4] = y1 * uu[0];
     double dh0 = uu[4];
-    // al = ALOSS / dh^2 and ce = 2 dh al / (3 k T) as one quotient (loss path)
+    double al = aloss(y0, dh, dh0, T) / (dh * dh);
     double tprime = 10.;
-    double ce = (2. * aloss(y0, dh, dh0, T)) / (3. * boltzm * T * dh);
-    double t1 = tprime + (T - tprime) * exp(-(ce * dt));
+    double ce = (2. * dh * al) / (3. * boltzm * T);
+    double t1 = tprime + (T - tprime) * libm::wexp(-(ce * dt));
     t1 = dmax(t1, 0.1 * T);
     t1 = dmin(t1, 10. * T);
     t1 = dmax(t1, tprime);
@@ -351,7 +341,7 @@ __device__ __forceinline__ void cool_dmc(const wx_params *__restrict__ p, double
     double Aloss = cooldmc(p, T);
     double dens = q[0];
     double Ce = (Aloss * (dens * dens)) / (Eth0 * p->Psc);
-    double pn = q[3] * exp(-(Ce * dt));
+    double pn = q[3] * libm::wexp(-(Ce * dt));
     uu[3] = uu[3] - Eth0 + p->cv * pn;
   }
 }
@@ -381,7 +371,7 @@ __device__ __forceinline__ void jet_state(const wx_params *__restrict__ p, doubl
 
 __device__ __forceinline__ double jet_velocity(const wx_params *__restrict__ p, double time) {
   const double pi = 3.141592653589793;  // acos(-1.)
-  return p->jet[6] + p->jet[7] * sin(2. * pi * time / p->jet[8]);
+  return p->jet[6] + p->jet[7] * libm::wsin(2. * pi * time / p->jet[8]);
 }
 
 // The source speed at `time`: the value the dt reduction stored for that
