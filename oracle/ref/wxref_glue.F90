@@ -10,8 +10,6 @@ module wxref_glue
   use parameters
   use globals
   implicit none
-  integer, allocatable, save :: pend_bounds(:,:)
-  integer, save :: pend_n = 0
   logical, save :: mpi_up = .false.
 contains
 
@@ -44,8 +42,6 @@ contains
     allocate(extbounds(1,5))
     nexternal = 0
 #endif
-    if (allocated(pend_bounds)) deallocate(pend_bounds)
-    allocate(pend_bounds(4*nblocks,3))
     u = 0.; up = 0.; primit = 0.
     lp = 0; leaf = .false.; icoord = 0
     innerbounds = 0
@@ -116,17 +112,43 @@ contains
     cnbleafs = nbleafs
   end subroutine wxr_get
 
+  ! install a given list as innerbounds (the reference's locatebounds
+  ! reallocates innerbounds to nbleafs*4 rows; grow it back when needed)
   subroutine wxr_set_bounds(cb, n) bind(C, name="wxr_set_bounds")
     integer(c_int), value :: n
     integer(c_int), intent(in) :: cb(3,n)
     integer :: k
+    if (size(innerbounds, 1) < max(n, 1)) then
+       deallocate(innerbounds)
+       allocate(innerbounds(max(n, 4*nblocks),3))
+    end if
     do k = 1, n
-       pend_bounds(k,:) = cb(:,k)
+       innerbounds(k,:) = cb(:,k)
     end do
-    pend_n = n
-    innerbounds(1:n,:) = pend_bounds(1:n,:)
     ninternal = n
   end subroutine wxr_set_bounds
+
+  ! the reference's own locatebounds; returns ninternal, copies the list
+  function wxr_locatebounds(time, cb, nmax) bind(C, name="wxr_locatebounds") result(n)
+    real(c_double), value :: time
+    integer(c_int), value :: nmax
+    integer(c_int), intent(out) :: cb(3,nmax)
+    integer(c_int) :: n
+    real :: t
+    integer :: k
+    t = time
+    call locatebounds(t)
+    n = ninternal
+    do k = 1, min(n, nmax)
+       cb(:,k) = innerbounds(k,:)
+    end do
+  end function wxr_locatebounds
+
+  ! the reference's own admesh (markref, critneighup, refine by level,
+  ! critneighdown, coarsen by level)
+  subroutine wxr_admesh() bind(C, name="wxr_admesh")
+    call admesh
+  end subroutine wxr_admesh
 
   subroutine wxr_set_jet(cj) bind(C, name="wxr_set_jet")
     use user_mod
@@ -348,17 +370,3 @@ contains
   end subroutine wxr_coarse
 end module wxref_glue
 
-!-----------------------------------------------------------------------
-! Harness hook: the reference's tstep calls locatebounds at its top
-! (src/tstep.f90:14).  The reference's locatebounds.f90 needs an ifort
-! extension (not(logical)), so here the list is installed from the one
-! handed over through wxr_set_bounds.
-!-----------------------------------------------------------------------
-subroutine locatebounds(time)
-  use wxref_glue, only : pend_bounds, pend_n
-  use globals
-  implicit none
-  real, intent(in) :: time
-  ninternal = pend_n
-  innerbounds(1:pend_n,:) = pend_bounds(1:pend_n,:)
-end subroutine locatebounds

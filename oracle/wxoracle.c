@@ -1019,27 +1019,17 @@ void wxo_updatelpdown(const wxo_params *p, wxo_mesh *m, int dad, int s1, int s2,
   if (nb1 != -1 && !m->leaf[nb1]) { nb1 = LP(m, nb1, 3); SETLP(m, nb1, 7, dad); SETLP(m, LP(m, nb1, 6), 7, dad); }
 }
 
-/* src/admesh.f90:4-111 with markref (src/markref.f90:4-85), critneighup
- * (src/critneighup.f90:4-62) and critneighdown (src/critneighdown.f90:4-56),
- * MPIP forms with one rank.  Returns WXO_ERR_BLOCKS on the :44-47 stop. */
-int wxo_admesh(const wxo_params *p, wxo_mesh *m) {
-  int n = m->cap + 1;
-  int32_t *up = (int32_t *)calloc((size_t)n, sizeof(int32_t));
-  int32_t *dn = (int32_t *)calloc((size_t)n, sizeof(int32_t));
-  /* markref */
+/* src/markref.f90:22-27: criteria over the leaves; flags by block ID,
+ * cleared first (MPIP with one rank: the flag exchange :51-78 is a no-op) */
+void wxo_markref(const wxo_params *p, const wxo_mesh *m, int32_t *up, int32_t *dn) {
+  for (int nb = 0; nb <= m->cap; ++nb) { up[nb] = 0; dn[nb] = 0; }
   for (int nb = 1; nb <= m->nbmax; ++nb)
     if (m->leaf[nb] && LP(m, nb, 1) >= 1) wxo_criteria(p, m, nb, up, dn);
-  int err = wxo_admesh_flags(p, m, up, dn);
-  free(up);
-  free(dn);
-  return err;
 }
 
-/* admesh after markref, from given flags (irefup/irefdown by block ID). */
-int wxo_admesh_flags(const wxo_params *p, wxo_mesh *m, int32_t *up, int32_t *dn) {
-  int err = 0;
-  int nbmaxp = m->nbmax, nbmaxp1 = m->nbmax;
-  /* critneighup */
+/* src/critneighup.f90:4-62: a block flagged for refinement at level nl
+ * flags its coarser neighbours, levels nlevs-1 down to 1 */
+void wxo_critneighup(const wxo_params *p, const wxo_mesh *m, int32_t *up) {
   for (int nl = p->nlevs - 1; nl >= 1; --nl)
     for (int nb = 1; nb <= m->nbmax; ++nb)
       if (up[nb] && LP(m, nb, 1) == nl)
@@ -1047,7 +1037,38 @@ int wxo_admesh_flags(const wxo_params *p, wxo_mesh *m, int32_t *up, int32_t *dn)
           int n1 = LP(m, nb, f);
           if (n1 != -1 && LP(m, n1, 1) < LP(m, nb, 1)) up[n1] = 1;
         }
-  /* refine by level */
+}
+
+/* src/critneighdown.f90:4-56: no coarsening next to a non-leaf neighbour */
+void wxo_critneighdown(const wxo_params *p, const wxo_mesh *m, int32_t *dn) {
+  (void)p;
+  for (int nb = 1; nb <= m->nbmax; ++nb)
+    if (dn[nb])
+      for (int f = 5; f <= 8; ++f) {
+        int n1 = LP(m, nb, f);
+        if (n1 != -1 && !m->leaf[n1]) dn[nb] = 0;
+      }
+}
+
+/* src/admesh.f90:4-111, MPIP forms with one rank.  Returns WXO_ERR_BLOCKS
+ * on the :44-47 stop. */
+int wxo_admesh(const wxo_params *p, wxo_mesh *m) {
+  int n = m->cap + 1;
+  int32_t *up = (int32_t *)calloc((size_t)n, sizeof(int32_t));
+  int32_t *dn = (int32_t *)calloc((size_t)n, sizeof(int32_t));
+  wxo_markref(p, m, up, dn);
+  int err = wxo_admesh_flags(p, m, up, dn);
+  free(up);
+  free(dn);
+  return err;
+}
+
+/* admesh after markref (:22), from given flags (irefup/irefdown by block ID). */
+int wxo_admesh_flags(const wxo_params *p, wxo_mesh *m, int32_t *up, int32_t *dn) {
+  int err = 0;
+  int nbmaxp = m->nbmax, nbmaxp1 = m->nbmax;
+  wxo_critneighup(p, m, up);                                  /* :24 */
+  /* refine by level :28-69 */
   for (int nl = 1; nl <= p->nlevs - 1 && !err; ++nl)
     for (int nb = 1; nb <= nbmaxp1; ++nb) {
       if (!(LP(m, nb, 1) == nl && up[nb])) continue;
@@ -1065,14 +1086,8 @@ int wxo_admesh_flags(const wxo_params *p, wxo_mesh *m, int32_t *up, int32_t *dn)
       wxo_updatelpup(p, m, nb, s1, s2, s3, s4, up, dn);
     }
   if (!err) {
-    /* critneighdown */
-    for (int nb = 1; nb <= m->nbmax; ++nb)
-      if (dn[nb])
-        for (int f = 5; f <= 8; ++f) {
-          int n1 = LP(m, nb, f);
-          if (n1 != -1 && !m->leaf[n1]) dn[nb] = 0;
-        }
-    /* coarsen by level */
+    wxo_critneighdown(p, m, dn);                              /* :72 */
+    /* coarsen by level :75-105 */
     for (int nl = p->nlevs; nl >= 2; --nl)
       for (int nb = 1; nb <= nbmaxp1; ++nb) {
         if (!(dn[nb] && LP(m, nb, 4) == 1 && LP(m, nb, 1) == nl)) continue;

@@ -103,6 +103,9 @@ void   wxo_updatelpup(const wxo_params *p, wxo_mesh *m, int nb, int s1, int s2, 
                       int32_t *irefup, int32_t *irefdown);
 void   wxo_coarse(const wxo_params *p, wxo_mesh *m, int dad, int s1, int s2, int s3, int s4);
 void   wxo_updatelpdown(const wxo_params *p, wxo_mesh *m, int dad, int s1, int s2, int s3, int s4);
+void   wxo_markref(const wxo_params *p, const wxo_mesh *m, int32_t *irefup, int32_t *irefdown);
+void   wxo_critneighup(const wxo_params *p, const wxo_mesh *m, int32_t *irefup);
+void   wxo_critneighdown(const wxo_params *p, const wxo_mesh *m, int32_t *irefdown);
 int    wxo_admesh(const wxo_params *p, wxo_mesh *m);
 int    wxo_admesh_flags(const wxo_params *p, wxo_mesh *m, int32_t *irefup, int32_t *irefdown);
 void   wxo_basegrid(const wxo_params *p, wxo_mesh *m);

@@ -119,6 +119,10 @@ class Oracle:
         L.wxo_coarse.argtypes = [P, M] + [ctypes.c_int] * 5
         L.wxo_updatelpdown.argtypes = [P, M] + [ctypes.c_int] * 5
         L.wxo_admesh.argtypes = [P, M]
+        L.wxo_markref.argtypes = [P, M, _ip, _ip]
+        L.wxo_admesh_flags.argtypes = [P, M, _ip, _ip]
+        L.wxo_critneighup.argtypes = [P, M, _ip]
+        L.wxo_critneighdown.argtypes = [P, M, _ip]
         L.wxo_basegrid.argtypes = [P, M]
         L.wxo_run.argtypes = [P, M, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp]
         L.wxo_hilbert2d.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int]
@@ -228,6 +232,23 @@ class Oracle:
     def admesh(self, params, m):
         return self.L.wxo_admesh(self._p(params), ctypes.byref(m.s))
 
+    def markref(self, params, m):
+        """src/markref.f90: refinement / coarsening flags (by block ID, 0..cap)."""
+        up = np.zeros(m.cap + 1, dtype=np.int32)
+        dn = np.zeros(m.cap + 1, dtype=np.int32)
+        self.L.wxo_markref(self._p(params), ctypes.byref(m.s), iptr(up), iptr(dn))
+        return up, dn
+
+    def admesh_flags(self, params, m, up, dn):
+        """admesh (src/admesh.f90:24-105) from given markref flags."""
+        return self.L.wxo_admesh_flags(self._p(params), ctypes.byref(m.s), iptr(up), iptr(dn))
+
+    def critneighup(self, params, m, up):
+        self.L.wxo_critneighup(self._p(params), ctypes.byref(m.s), iptr(up))
+
+    def critneighdown(self, params, m, dn):
+        self.L.wxo_critneighdown(self._p(params), ctypes.byref(m.s), iptr(dn))
+
     def hilbert2d(self, x, y, bits):
         """src/loadbalance.f90:251-306: the Hilbert key of (x, y) over `bits` bits."""
         return int(self.L.wxo_hilbert2d(int(x), int(y), int(bits)))
@@ -315,6 +336,9 @@ class RefLib:
         L.wxr_critneighdown.argtypes = [_ip]
         L.wxr_refine.argtypes = [ctypes.c_int] * 5 + [_ip, _ip]
         L.wxr_coarse.argtypes = [ctypes.c_int] * 5
+        L.wxr_locatebounds.argtypes = [ctypes.c_double, _ip, ctypes.c_int]
+        L.wxr_locatebounds.restype = ctypes.c_int
+        L.wxr_admesh.argtypes = []
         c = np.zeros(12)
         L.wxr_consts(dptr(c))
         self.consts = dict(cv=c[0], gamma=c[1], Tempsc=c[2], Psc=c[3], tsc=c[4], cfl=c[5],
@@ -349,6 +373,18 @@ class RefLib:
 
     def flags(self):
         return (np.zeros(self.nblocks + 1, dtype=np.int32), np.zeros(self.nblocks + 1, dtype=np.int32))
+
+    def locatebounds(self, time=0.0):
+        """The reference's own locatebounds (src/locatebounds.f90:29-540) on the
+        state last put: innerbounds(1:ninternal, 1:3) as an (n, 3) array."""
+        cap = 4 * self.nblocks
+        b = np.zeros((cap, 3), dtype=np.int32)
+        n = self.L.wxr_locatebounds(float(time), iptr(b), cap)
+        return b[:n].copy()
+
+    def admesh(self):
+        """The reference's own admesh (src/admesh.f90:4-111) on the state last put."""
+        self.L.wxr_admesh()
 
     def uprim(self, uu):
         uu = np.ascontiguousarray(uu, dtype=np.float64)

@@ -66,3 +66,40 @@ def test_steps_golden(variant, nsteps):
         assert np.array_equal(np.array(lv, dtype=np.int32), G[f"step_{variant}_leaves"])
         np.testing.assert_allclose(m.u[lv][:, :, 2:18, 2:18].sum(axis=(2, 3)), G[f"step_{variant}_sums"],
                                    rtol=1e-13)
+
+
+# --- mesh layer: vectors from the reference's own locatebounds / admesh /
+# time loop (tests/golden/make_golden_mesh.py) ---
+import mesh_inputs as MI  # noqa: E402
+from harness import oracle_mesh  # noqa: E402
+
+GM = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_mesh_vectors.npz"))
+
+
+@pytest.mark.parametrize("variant,seed", MI.LB_CASES)
+def test_locatebounds_order_golden(variant, seed):
+    cfg, pp, m = MI.random_mesh(O, variant, seed)
+    n = O.locatebounds(pp, m)
+    assert np.array_equal(m.bounds[:n], GM[f"lb_{variant}_{seed}"])
+
+
+@pytest.mark.parametrize("variant,steps", MI.LOOP_CASES)
+def test_time_loop_golden(variant, steps):
+    """The oracle's time loop (timestep, tstep, admesh) against the
+    reference's: dt and a digest of the tree and leaf state, every step."""
+    cfg = MI.cfg_for(variant)
+    pp, m = oracle_mesh(O, cfg, 3000)
+    if cfg.nlevs > 1:
+        for _ in range(3):
+            assert O.admesh(pp, m) == 0
+    t = 0.0
+    dts, dig = GM[f"loop_{variant}_dt"], GM[f"loop_{variant}_digest"]
+    assert len(dts) == steps
+    for s in range(steps):
+        dt = O.timestep(pp, m)
+        assert dt == dts[s], (variant, s)
+        t += dt
+        assert O.tstep(pp, m, t, dt) == 0
+        if cfg.nlevs > 1:
+            assert O.admesh(pp, m) == 0
+        assert np.array_equal(MI.state_digest(m), dig[s]), (variant, s)

@@ -28,7 +28,7 @@ def cfg_for(variant):
     if variant == "c1":
         return P.config_c1()
     if variant == "c2":
-        return P.config_c2(nblocks=4096)
+        return P.config_c2(nblocks=4096, tol=0.05)   # criteria's own threshold, src/criteria.f90:11
     if variant == "c3":
         return P.config_c3(nblocks=4096, nbrootx=16, nbrooty=4)
     base = dict(name=variant, neq=5, nlevs=3, nbrootx=4, nbrooty=4, xmax=1.0, ymax=1.0, xphys=P.PC,
