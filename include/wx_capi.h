@@ -299,6 +299,17 @@ int wx_sur_train(wx_sur_model *m, const float *x, const float *t, int64_t n, int
 int wx_set_surrogate(wx_ctx *ctx, const wx_sur_model *m);
 int wx_ens_set_surrogate(wx_ens *ens, const wx_sur_model *m);
 int wx_sur_counters(wx_ctx *ctx, int32_t reset, int64_t *cells, int64_t *fallbacks);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): wx_sur_infer with the model (a wx_sur_model), the
+ * inputs x[n][4] and the outputs y[n][2] all in device memory, launched on
+ * `stream` (a hipStream_t; NULL = the default stream) and not waited for.
+ * The device form of the model is built by kernels in a fixed static device
+ * buffer of 160 KiB per GPU (held until the process ends) instead of memory
+ * allocated per call; n needs no scratch and is not limited by it.  The
+ * buffer keeps the last call's model; calls that overlap on one GPU race on
+ * it.  Results equal wx_sur_infer's bit for bit. */
+int wx_sur_infer_noalloc(const wx_sur_model *d_model, const float *d_x, int64_t n, float *d_y, void *stream);
 int wx_ens_sur_counters(wx_ens *ens, int32_t reset, int64_t *cells, int64_t *fallbacks);
 
 #ifdef __cplusplus

@@ -33,7 +33,7 @@ EXPORTS = [
     "wx_ens_profile", "wx_ens_info", "wx_ens_create_groups", "wx_ens_groups",
     "wx_shape", "wx_write_bin", "wx_read_bin", "wx_ens_shape", "wx_ens_write_bin",
     "wx_sur_pairs", "wx_sur_features", "wx_sur_infer", "wx_sur_grad", "wx_sur_train", "wx_set_surrogate",
-    "wx_ens_set_surrogate", "wx_sur_counters", "wx_ens_sur_counters", "wx_lb_plan",
+    "wx_ens_set_surrogate", "wx_sur_counters", "wx_ens_sur_counters", "wx_lb_plan", "wx_sur_infer_noalloc",
 ]
 
 _fp = ctypes.POINTER(ctypes.c_float)
@@ -110,6 +110,7 @@ def load_library(path: str = LIBPATH) -> ctypes.CDLL:
     L.wx_sur_pairs.argtypes = [ctypes.POINTER(WxParams), _dp, ctypes.c_int64, _dp]
     L.wx_sur_features.argtypes = [ctypes.POINTER(WxParams), _dp, ctypes.c_int64, _fp, _fp]
     L.wx_sur_infer.argtypes = [SM, _fp, ctypes.c_int64, _fp]
+    L.wx_sur_infer_noalloc.argtypes = [P, P, ctypes.c_int64, P, P]
     L.wx_sur_grad.argtypes = [SM, _fp, _fp, ctypes.c_int64, ctypes.c_float, ctypes.c_float, _fp, _fp]
     L.wx_sur_train.argtypes = [SM, _fp, _fp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
                                ctypes.c_float, ctypes.c_float, ctypes.c_float, _fp]
