@@ -1,7 +1,10 @@
 // Kernel launch stand-ins for the host-only build of wx_host.cpp.  The
 // statistics kernels write refinement flags from a table the test sets
 // (wxsim_set_flags), so the host's admesh can be driven with chosen flags;
-// the mesh-op launches record the ops they were given.  TEST SCAFFOLDING.
+// the mesh-op launches record the ops they were given.  Every launch goes
+// through hipsim::launch with the pointers it was given, so a launch on a
+// capturing stream becomes a graph node (hip/hip_runtime.h).  TEST
+// SCAFFOLDING.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -45,14 +48,16 @@ static void write_flags(const wx_params *P, const LeafRec *leaves, int n, signed
 }
 int launch_build_queue(const QueueHdr *hdr, const int32_t *inst_off, const int32_t *leafbase,
                        const LeafRec *reg_rec, const int2 *reg_ops2, const int2 *reg_ops1, LeafRec *rec,
-                       int2 *ops2, int2 *ops1, int, hipStream_t) {
-  for (int l = 0; l < hdr->nleaf; ++l) {
-    int i = 0;
-    while (i + 1 < hdr->ninst && inst_off[i + 1] <= l) ++i;
-    const size_t src = (size_t)leafbase[i] + (size_t)(l - inst_off[i]);
-    rec[l] = reg_rec[src];
-    for (int k = 0; k < 8; ++k) { ops2[(size_t)l * 8 + k] = reg_ops2[src * 8 + k]; ops1[(size_t)l * 8 + k] = reg_ops1[src * 8 + k]; }
-  }
+                       int2 *ops2, int2 *ops1, int, hipStream_t st) {
+  hipsim::launch(st, {hdr, inst_off, leafbase, reg_rec, reg_ops2, reg_ops1, rec, ops2, ops1}, [=] {
+    for (int l = 0; l < hdr->nleaf; ++l) {
+      int i = 0;
+      while (i + 1 < hdr->ninst && inst_off[i + 1] <= l) ++i;
+      const size_t src = (size_t)leafbase[i] + (size_t)(l - inst_off[i]);
+      rec[l] = reg_rec[src];
+      for (int k = 0; k < 8; ++k) { ops2[(size_t)l * 8 + k] = reg_ops2[src * 8 + k]; ops1[(size_t)l * 8 + k] = reg_ops1[src * 8 + k]; }
+    }
+  });
   return 0;
 }
 // A one-entry-per-workgroup launch must cover the whole queue.
@@ -63,67 +68,77 @@ static void check_shape(QueueRef q, int grid, int bs, int persist) {
     abort();
   }
 }
-int launch_stageA(const wx_params *, int, QueueRef q, int grid, int bs, int persist, const double *, double *,
-                  DevScalars *, hipStream_t) {
-  check_shape(q, grid, bs, persist);
+int launch_stageA(const wx_params *P, int, QueueRef q, int grid, int bs, int persist, const double *U, double *Uh,
+                  DevScalars *S, hipStream_t st) {
+  hipsim::launch(st, {P, q.hdr, q.rec, U, Uh, S}, [=] { check_shape(q, grid, bs, persist); });
   return 0;
 }
-int launch_stageB(const wx_params *, int, QueueRef q, int grid, int bs, int persist, const int2 *, const BlockRec *,
-                  double *, const double *, DevScalars *, hipStream_t) {
-  check_shape(q, grid, bs, persist);
+int launch_stageB(const wx_params *P, int, QueueRef q, int grid, int bs, int persist, const int2 *ops,
+                  const BlockRec *binfo, double *U, const double *Uh, DevScalars *S, hipStream_t st) {
+  hipsim::launch(st, {P, q.hdr, q.rec, ops, binfo, U, Uh, S}, [=] { check_shape(q, grid, bs, persist); });
   return 0;
 }
-int launch_fillI_cool(const wx_params *P, int, QueueRef q, int grid, int bs, int persist, const int2 *,
-                      const BlockRec *, const double *, double *, double *stat, int8_t *flag, DevScalars *S,
-                      FlagOut fo, hipStream_t) {
-  check_shape(q, grid, bs, persist);
-  write_flags(P, q.rec, q.hdr->nleaf, (signed char *)flag, stat);
-  if (fo.flagq) {
-    for (int l = 0; l < q.hdr->nleaf; ++l) fo.flagq[fo.inst_off[q.rec[l].inst] + q.rec[l].lidx] = flag[q.rec[l].id];
-    for (int i = 0; i < fo.ninst; ++i) fo.Sout[i] = S[i];
-  }
+int launch_fillI_cool(const wx_params *P, int, QueueRef q, int grid, int bs, int persist, const int2 *ops,
+                      const BlockRec *binfo, const double *Uin, double *Uout, double *stat, int8_t *flag,
+                      DevScalars *S, FlagOut fo, hipStream_t st) {
+  hipsim::launch(st, {P, q.hdr, q.rec, ops, binfo, Uin, Uout, stat, flag, S, fo.inst_off, fo.flagq, fo.Sout}, [=] {
+    check_shape(q, grid, bs, persist);
+    write_flags(P, q.rec, q.hdr->nleaf, (signed char *)flag, stat);
+    if (fo.flagq) {
+      for (int l = 0; l < q.hdr->nleaf; ++l) fo.flagq[fo.inst_off[q.rec[l].inst] + q.rec[l].lidx] = flag[q.rec[l].id];
+      for (int i = 0; i < fo.ninst; ++i) fo.Sout[i] = S[i];
+    }
+  });
   return 0;
 }
-int launch_blockstat(const wx_params *P, int, const LeafRec *leaves, int nleaf, const double *, double *stat,
-                     int8_t *flag, hipStream_t) {
-  write_flags(P, leaves, nleaf, (signed char *)flag, stat);
+int launch_blockstat(const wx_params *P, int, const LeafRec *leaves, int nleaf, const double *U, double *stat,
+                     int8_t *flag, hipStream_t st) {
+  hipsim::launch(st, {P, leaves, U, stat, flag}, [=] { write_flags(P, leaves, nleaf, (signed char *)flag, stat); });
   return 0;
 }
-int launch_dtreduce(const wx_params *, const LeafRec *, const double *, const int32_t *, int ninst,
-                    DevScalars *S, int adv, hipStream_t) {
-  for (int i = 0; i < ninst; ++i) {
-    S[i].dt = 1e-3;
-    if (adv) S[i].time += S[i].dt;
-  }
+int launch_dtreduce(const wx_params *P, const LeafRec *rec, const double *stat, const int32_t *inst_off, int ninst,
+                    DevScalars *S, int adv, hipStream_t st) {
+  hipsim::launch(st, {P, rec, stat, inst_off, S}, [=] {
+    for (int i = 0; i < ninst; ++i) {
+      S[i].dt = 1e-3;
+      if (adv) S[i].time += S[i].dt;
+    }
+  });
   return 0;
 }
 int launch_gather_flags(QueueRef q, const int32_t *inst_off, int, const int8_t *flag, int8_t *flagq, const DevScalars *S,
-                        DevScalars *Sout, int ninst, hipStream_t) {
-  for (int l = 0; l < q.hdr->nleaf; ++l) flagq[inst_off[q.rec[l].inst] + q.rec[l].lidx] = flag[q.rec[l].id];
-  if (Sout) for (int i = 0; i < ninst; ++i) Sout[i] = S[i];
+                        DevScalars *Sout, int ninst, hipStream_t st) {
+  hipsim::launch(st, {q.hdr, q.rec, inst_off, flag, flagq, S, Sout}, [=] {
+    for (int l = 0; l < q.hdr->nleaf; ++l) flagq[inst_off[q.rec[l].inst] + q.rec[l].lidx] = flag[q.rec[l].id];
+    if (Sout) for (int i = 0; i < ninst; ++i) Sout[i] = S[i];
+  });
   return 0;
 }
-int launch_meshops(const wx_params *P, int, const MeshOp *ops, int nops, double *, double *stat, int8_t *flag,
-                   hipStream_t) {
-  std::lock_guard<std::mutex> g(g_mu);
-  for (int k = 0; k < nops; ++k) {
-    g_ops.push_back(ops[k]);
-    if (!stat) continue;
-    LeafRec r[4] = {};
-    const int ids[4] = {ops[k].s1, ops[k].s2, ops[k].s3, ops[k].s4};
-    const int n = ops[k].kind == 0 ? 4 : 1;
-    for (int s = 0; s < n; ++s) {
-      r[s].id = ops[k].kind == 0 ? ids[s] : ops[k].a;
-      r[s].level = ops[k].level;
-      r[s].inst = ops[k].inst;
+int launch_meshops(const wx_params *P, int, const MeshOp *ops, int nops, double *U, double *stat, int8_t *flag,
+                   hipStream_t st) {
+  hipsim::launch(st, {P, ops, U, stat, flag}, [=] {
+    std::lock_guard<std::mutex> g(g_mu);
+    for (int k = 0; k < nops; ++k) {
+      g_ops.push_back(ops[k]);
+      if (!stat) continue;
+      LeafRec r[4] = {};
+      const int ids[4] = {ops[k].s1, ops[k].s2, ops[k].s3, ops[k].s4};
+      const int n = ops[k].kind == 0 ? 4 : 1;
+      for (int s = 0; s < n; ++s) {
+        r[s].id = ops[k].kind == 0 ? ids[s] : ops[k].a;
+        r[s].level = ops[k].level;
+        r[s].inst = ops[k].inst;
+      }
+      write_flags(P, r, n, (signed char *)flag, stat);
     }
-    write_flags(P, r, n, (signed char *)flag, stat);
-  }
+  });
   return 0;
 }
 int launch_transpose(int, int, const double *, double *, int, int, hipStream_t) { return 0; }
-int launch_h2d(const CopyDesc *d, int n, hipStream_t) {
-  for (int k = 0; k < n; ++k) memmove(d[k].dst, d[k].src, d[k].bytes);
+int launch_h2d(const CopyDesc *d, int n, hipStream_t st) {
+  hipsim::launch(st, {d}, [=] {
+    for (int k = 0; k < n; ++k) memmove(d[k].dst, d[k].src, d[k].bytes);
+  });
   return 0;
 }
 int launch_fillI_cool_sur(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const int2 *ops,

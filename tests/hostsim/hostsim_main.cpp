@@ -14,6 +14,7 @@
 
 #include "../../include/wx_capi.h"
 #include "../../oracle/wxoracle.h"
+#include "hip/hip_runtime.h"
 
 extern "C" void wxsim_set_flags(const signed char *f, int n);
 extern "C" int wxsim_ops(int *out, int cap);
@@ -153,7 +154,13 @@ static int run_ensemble(int rounds, unsigned seed, int groups) {
   std::mt19937 rng(seed);
   std::uniform_real_distribution<double> U01(0.0, 1.0);
   int total_ops = 0;
+  // steps replayed from captured graphs (re-captured as the queues grow and
+  // shrink); a third of the way the fill switches to a persistent grid and
+  // back at two thirds, which drops and re-captures the graphs again
+  if (wx_ens_set_graph(e, 1)) { fprintf(stderr, "set_graph failed\n"); return 2; }
   for (int r = 0; r < rounds; ++r) {
+    if (r == rounds / 3 && wx_ens_set_launch(e, 2, 64, 256)) { fprintf(stderr, "set_launch failed\n"); return 2; }
+    if (r == 2 * rounds / 3 && wx_ens_set_launch(e, 2, 0, 256)) { fprintf(stderr, "set_launch failed\n"); return 2; }
     std::vector<std::vector<signed char>> f(groups);
     for (int k = 0; k < groups; ++k) f[k].assign(gslots[k], 0);
     std::vector<std::vector<int32_t>> up(n), dn(n);
@@ -206,8 +213,8 @@ static int run_ensemble(int rounds, unsigned seed, int groups) {
       }
     }
   }
-  printf("hostsim ensemble: %d instances in %d groups, %d rounds, %d mesh ops -- trees identical\n", n, groups, rounds,
-         total_ops);
+  printf("hostsim ensemble: %d instances in %d groups, %d rounds, %d mesh ops, %d graph captures, %d replays -- "
+         "trees identical\n", n, groups, rounds, total_ops, hipsim::captures, hipsim::replays);
   wx_ens_destroy(e);
   return 0;
 }

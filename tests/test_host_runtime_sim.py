@@ -26,13 +26,17 @@ def test_host_tree_matches_oracle_under_asan(seed):
 def test_ensemble_trees_match_oracle_under_asan(groups):
     """Three meshes of different root layouts, depths and capacities advanced as
     one ensemble (worker threads on), in one group or in three groups stepped
-    from concurrent host threads: every instance's tree after every admesh
-    equals the oracle's for the same flags."""
+    from concurrent host threads, with the steps replayed from captured graphs
+    (re-captured as the queues grow and shrink, and after launch-shape
+    changes): every instance's tree after every admesh equals the oracle's for
+    the same flags, no allocation a live graph references is freed, and no
+    copy or memset is captured."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
     r = subprocess.run([os.path.join(HERE, "hostsim_main"), "120", "5", "ens", str(groups)], capture_output=True,
                        text=True, timeout=600, env={**os.environ, "WX_HOST_THREADS": "4"})
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
     assert "trees identical" in r.stdout
+    assert " 0 graph captures" not in r.stdout and " 0 replays" not in r.stdout, r.stdout
 
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not installed")
