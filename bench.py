@@ -55,101 +55,128 @@ L1_BOUND = 1e-3   # relative L1 density error the batched path is held to
 ALG_PASSES = {"stageA": 2, "stageB": 3, "fillI_cool": 2}
 
 
-def cpu_baseline_reference(cfg, seconds: float = 12.0):
-    """The reference's own Fortran (its hot-path sources built unmodified
-    into oracle/_ref/, one MPI rank, one core) stepping the same workload:
-    same initial mesh after the 4 initial admesh passes, then timestep +
-    tstep until ~`seconds` of CPU.  The reference's admesh is not part of
-    that build (its locatebounds/updatelpdown need an ifort extension), so
-    the mesh stays fixed and the list is installed once.  None when the
-    build is absent."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from wxoracle import MeshState, Oracle, RefLib, ref_available
-    from walicxe2d_amd import params as P
-    variant = {"c1_explosion": "c1", "c2_sn_dmc": "c2", "c3_jet_coolingh": "c3"}.get(cfg.name)
-    if variant is None or not ref_available(variant):
-        return None
+def _cpu_share() -> int:
+    """Host cores this process may use, at most 16 (a one-GPU host's share)."""
     try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _instance_cfg(workload: str, k: int):
+    """Instance k of a workload's CPU sample: for the jet, the batched
+    ensemble's variants (seeds 777+100+k); otherwise the workload itself."""
+    from walicxe2d_amd import params as P
+    if workload == "c3":
+        return P.config_c5(100 + k)
+    return P.CONFIGS[workload]()
+
+
+def _initial_mesh(O, cfg, cap):
+    """The workload's initial state on the oracle's mesh arrays (roots only)."""
+    from wxoracle import MeshState
+    from walicxe2d_amd import params as P
+    pp = cfg.params()
+    m = MeshState(cap, cfg.neq)
+    O.basegrid(pp, m)
+    u, lp, ic = P.initial_state(cfg)
+    n = lp.shape[0] - 1
+    m.u[1:n + 1] = u[1:]
+    for nb in range(1, n + 1):
+        m.primit[nb] = O.calcprim(pp, m.u[nb])
+    return pp, m
+
+
+def cpu_child(kind: str, workload: str, k: int, seconds: float):
+    """One CPU process of the baseline: instance k of the workload from its
+    initial state through the 4 initial admesh passes (src/main.f90:60-63)
+    and then the time loop (timestep, time += dt, tstep, admesh;
+    src/main.f90:87-98) for about `seconds`.  kind "reference": the
+    reference's own Fortran (its sources built unmodified into oracle/_ref/,
+    one MPI rank) running its own loop, admesh and locatebounds included;
+    kind "port": the C oracle."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from wxoracle import Oracle, RefLib
+    cfg = _instance_cfg(workload, k)
+    O = Oracle()
+    amr = cfg.nlevs > 1
+    if kind == "reference":
+        variant = {"c1": "c1", "c2": "c2", "c3": "c3"}[workload]
         R = RefLib(variant)
-    except OSError:
-        return None
-    O = Oracle()
-    pp = cfg.params()
-    m = MeshState(R.nblocks - 1, cfg.neq)
-    O.basegrid(pp, m)
-    u, lp, ic = P.initial_state(cfg)
-    n = lp.shape[0] - 1
-    m.u[1:n + 1] = u[1:]
-    for nb in range(1, n + 1):
-        m.primit[nb] = O.calcprim(pp, m.u[nb])
-    if cfg.nlevs > 1:
-        for _ in range(4):
-            O.admesh(pp, m)
-    O.locatebounds(pp, m)
-    R.setup(pp)
-    R.put(m)
-    leaves = len(m.leaves())
-    t = 0.0
+        pp, m = _initial_mesh(O, cfg, R.nblocks - 1)
+        R.setup(pp)
+        R.put(m)
+        if amr:
+            for _ in range(4):
+                R.admesh()
+    else:
+        pp, m = _initial_mesh(O, cfg, min(cfg.nblocks, 8000))   # a bounded sample's mesh fits 8000 IDs
+        if amr:
+            for _ in range(4):
+                O.admesh(pp, m)
+    t = zones = 0.0
     steps = 0
     t0 = time.perf_counter()
     while time.perf_counter() - t0 < seconds:
-        dt = R.L.wxr_timestep()
-        t += dt
-        R.L.wxr_tstep(t, dt)
+        if kind == "reference":
+            zones += R.L.wxr_nbleafs() * 256.0
+            dt = R.L.wxr_timestep()
+            t += dt
+            R.L.wxr_tstep(t, dt)
+            if amr:
+                R.admesh()
+        else:
+            t, _, z, err = O.run(pp, m, 1, t, do_admesh=amr)
+            if err:
+                break
+            zones += z
         steps += 1
     el = time.perf_counter() - t0
-    zones = float(steps) * leaves * 256
-    return {"value": zones / el, "unit": "zone-updates/s", "cores": 1, "kind": "reference",
-            "sample": f"{steps} steps from t=0 on the initial mesh ({leaves} leaves, fixed: the reference's admesh is "
-                      f"not in the build; {zones:.0f} zone-updates, {el:.1f} s), reference timestep + tstep compiled "
-                      "from its own sources with flang -O2, one MPI rank"}
+    leaves = R.L.wxr_nbleafs() if kind == "reference" else m.nbleafs
+    return {"zones": zones, "seconds": el, "steps": steps, "leaves_end": int(leaves)}
 
 
-def _reference_in_child(workload: str):
-    """cpu_baseline_reference in a child process (the reference build
-    initialises MPI; a failure there must not end the bench)."""
+def cpu_baseline(kind: str, workload: str, seconds: float = 15.0):
+    """The CPU baseline on all the host cores this process may use (at most
+    16): one process per core, each running its own instance of the
+    workload (cpu_child), started together; the throughput is the sum of
+    their zone-updates over the longest process's wall time.  None when the
+    reference build is absent (kind "reference") or a process fails."""
     import subprocess
-    try:
-        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-reference-only", "--workload", workload],
-                           capture_output=True, text=True, timeout=120)
-        lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
-        return json.loads(lines[-1]) if r.returncode == 0 and lines else None
-    except (subprocess.TimeoutExpired, ValueError):
-        return None
-
-
-def cpu_baseline(cfg, seconds: float = 12.0):
-    """The C oracle (single thread) on a bounded sample of the same workload:
-    same initial mesh, same steps (incl. admesh), until ~`seconds` of CPU."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from wxoracle import MeshState, Oracle
-    from walicxe2d_amd import params as P
-    O = Oracle()
-    pp = cfg.params()
-    m = MeshState(cfg.nblocks, cfg.neq)
-    O.basegrid(pp, m)
-    u, lp, ic = P.initial_state(cfg)
-    n = lp.shape[0] - 1
-    m.u[1:n + 1] = u[1:]
-    for nb in range(1, n + 1):
-        m.primit[nb] = O.calcprim(pp, m.u[nb])
-    if cfg.nlevs > 1:
-        for _ in range(4):
-            O.admesh(pp, m)
-    t = 0.0
-    zones = 0.0
-    steps = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < seconds:
-        t, _, z, err = O.run(pp, m, 1, t, do_admesh=cfg.nlevs > 1)
-        if err:
-            break
-        zones += z
-        steps += 1
-    el = time.perf_counter() - t0
-    return {"value": zones / el, "unit": "zone-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} steps of the same workload from t=0 ({zones:.0f} zone-updates, "
-                      f"{el:.1f} s), C oracle built with gcc -O2, one thread"}
+    if kind == "reference":
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from wxoracle import ref_available
+        if workload not in ("c1", "c2", "c3") or not ref_available(workload):
+            return None
+    n = _cpu_share()
+    env = {**os.environ, "OMP_NUM_THREADS": "1"}
+    procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-child", kind, "--workload", workload,
+                               "--cpu-instance", str(k), "--cpu-seconds", str(seconds)],
+                              stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True, env=env)
+             for k in range(n)]
+    res = []
+    for pr in procs:
+        try:
+            out, _ = pr.communicate(timeout=seconds + 120)
+        except subprocess.TimeoutExpired:
+            pr.kill()
+            return None
+        lines = [ln for ln in out.splitlines() if ln.startswith("{")]
+        if pr.returncode != 0 or not lines:
+            return None
+        res.append(json.loads(lines[-1]))
+    zones = sum(r["zones"] for r in res)
+    el = max(r["seconds"] for r in res)
+    steps = sum(r["steps"] for r in res)
+    what = ("the reference's own Fortran (timestep, tstep, admesh and locatebounds compiled unmodified from its "
+            "sources with flang -O2, one MPI rank per process)" if kind == "reference" else
+            "the C oracle (gcc -O2, one thread per process)")
+    inst = ("the batched ensemble's jet variants (seeds 777+100..)" if workload == "c3" else "the workload")
+    return {"value": zones / el, "unit": "zone-updates/s", "cores": n, "kind": kind,
+            "sample": f"{n} processes on {n} host cores, each one instance of {inst} from t=0 through the "
+                      f"4 initial admesh passes, ~{seconds:.0f} s of its time loop: {steps} steps, {zones:.3e} "
+                      f"zone-updates in {el:.1f} s; {what}"}
 
 
 def roofline(prof, zones, neq, workload, peak=HBM_PEAK_GBS):
@@ -309,11 +336,12 @@ def main():
     ap.add_argument("--eager", action="store_true", help="no hipGraphs: every launch from the host")
     ap.add_argument("--c4-steps", type=int, default=300, help="steps of the 64-instance C4 comparison (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-reference-only", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-child", choices=["reference", "port"], help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-instance", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help=argparse.SUPPRESS)
     args = ap.parse_args()
-    if args.cpu_reference_only:
-        from walicxe2d_amd import params as P
-        print(json.dumps(cpu_baseline_reference(P.CONFIGS[args.workload]())), flush=True)
+    if args.cpu_child:
+        print(json.dumps(cpu_child(args.cpu_child, args.workload, args.cpu_instance, args.cpu_seconds)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: the library keeps every leaf block on one "
@@ -401,8 +429,8 @@ def main():
         out["c4"] = run_c4(args.c4_steps, 20, args.groups)
     if not args.no_cpu_baseline:
         # the reference's own code where its build is present, else the oracle
-        ref = _reference_in_child(args.workload)
-        port = cpu_baseline(P.CONFIGS[args.workload]())
+        ref = cpu_baseline("reference", args.workload)
+        port = cpu_baseline("port", args.workload)
         out["cpu_baseline"] = ref if ref is not None else port
         if ref is not None:
             out["cpu_baseline_port"] = port

@@ -144,6 +144,12 @@ contains
     end do
   end function wxr_locatebounds
 
+  ! leaf blocks of the reference's mesh (its nbleafs global)
+  function wxr_nbleafs() bind(C, name="wxr_nbleafs") result(n)
+    integer(c_int) :: n
+    n = nbleafs
+  end function wxr_nbleafs
+
   ! the reference's own admesh (markref, critneighup, refine by level,
   ! critneighdown, coarsen by level)
   subroutine wxr_admesh() bind(C, name="wxr_admesh")

@@ -339,6 +339,7 @@ class RefLib:
         L.wxr_locatebounds.argtypes = [ctypes.c_double, _ip, ctypes.c_int]
         L.wxr_locatebounds.restype = ctypes.c_int
         L.wxr_admesh.argtypes = []
+        L.wxr_nbleafs.restype = ctypes.c_int
         c = np.zeros(12)
         L.wxr_consts(dptr(c))
         self.consts = dict(cv=c[0], gamma=c[1], Tempsc=c[2], Psc=c[3], tsc=c[4], cfl=c[5],
