@@ -144,6 +144,15 @@ contains
     end do
   end function wxr_locatebounds
 
+  ! the reference's own output (src/output.f90, OUTBIN branch): BIN/pointers
+  ! and BIN/blocks files of rank 0 under the working directory
+  subroutine wxr_output(itprint) bind(C, name="wxr_output")
+    integer(c_int), value :: itprint
+    integer :: it
+    it = itprint
+    call output(it)
+  end subroutine wxr_output
+
   ! leaf blocks of the reference's mesh (its nbleafs global)
   function wxr_nbleafs() bind(C, name="wxr_nbleafs") result(n)
     integer(c_int) :: n

@@ -340,6 +340,7 @@ class RefLib:
         L.wxr_locatebounds.restype = ctypes.c_int
         L.wxr_admesh.argtypes = []
         L.wxr_nbleafs.restype = ctypes.c_int
+        L.wxr_output.argtypes = [ctypes.c_int]
         c = np.zeros(12)
         L.wxr_consts(dptr(c))
         self.consts = dict(cv=c[0], gamma=c[1], Tempsc=c[2], Psc=c[3], tsc=c[4], cfl=c[5],

@@ -5,6 +5,8 @@
 // Usage: hostsim_main <rounds> <seed> [ens].  With "ens", three instances of
 // different root layouts, depths and capacities run as one wx_ens ensemble
 // and each instance's tree is compared with its own oracle mesh.
+// hostsim_main 0 0 writebin STATE DIR uploads the state in the file STATE
+// and writes its BIN pair into DIR with wx_write_bin (write_bin_of_state).
 // TEST SCAFFOLDING.
 #include <cstdio>
 #include <cstdlib>
@@ -22,11 +24,13 @@ extern "C" void wxsim_set_group_flags(int group, const signed char *f, int n);
 
 static int run_ensemble(int rounds, unsigned seed, int groups);
 static int bin_roundtrip(wx_ctx *c, const wx_params &p, const std::string &dir);
+static int write_bin_of_state(const char *state, const char *dir);
 
 int main(int argc, char **argv) {
   const int rounds = argc > 1 ? atoi(argv[1]) : 200;
   const unsigned seed = argc > 2 ? (unsigned)atoi(argv[2]) : 1u;
   if (argc > 3 && std::string(argv[3]) == "ens") return run_ensemble(rounds, seed, argc > 4 ? atoi(argv[4]) : 1);
+  if (argc > 5 && std::string(argv[3]) == "writebin") return write_bin_of_state(argv[4], argv[5]);
   const int cap = 30000, nrx = 8, nry = 8, nlevs = 5, neq = 4;
   wx_params p{};
   p.neq = neq; p.nlevs = nlevs; p.nbrootx = nrx; p.nbrooty = nry; p.nblocks = cap;
@@ -279,5 +283,52 @@ static int bin_roundtrip(wx_ctx *c, const wx_params &p, const std::string &dir) 
   }
   wx_destroy(c2);
   printf("bin: %d blocks, %d leaves written and read back identical\n", nbmax, nbleafs);
+  return 0;
+}
+
+// ---------------------------------------------------------------------
+// writebin: STATE holds int32 neq, nblocks, nlevs, nbmax, nbleafs,
+// logical_true, then lp (8 per ID, IDs 1..nbmax), leaf (nbmax), icoord (2
+// per ID), then float64 tiles 0..nbmax [var][j][i]; the context gets
+// capacity nblocks, the state is uploaded and written by wx_write_bin as
+// DIR/pointers000.001.bin and DIR/blocks000.001.bin.
+static int write_bin_of_state(const char *state, const char *dir) {
+  FILE *f = fopen(state, "rb");
+  if (!f) { fprintf(stderr, "writebin: cannot open %s\n", state); return 2; }
+  int32_t h[6];
+  if (fread(h, 4, 6, f) != 6) { fclose(f); return 2; }
+  const int neq = h[0], cap = h[1], nlevs = h[2], nbmax = h[3], nbleafs = h[4], ltrue = h[5];
+  std::vector<int32_t> lpr((size_t)8 * nbmax), leaf(nbmax), icr((size_t)2 * nbmax);
+  std::vector<double> u((size_t)(nbmax + 1) * neq * 400);
+  bool ok = fread(lpr.data(), 4, lpr.size(), f) == lpr.size() && fread(leaf.data(), 4, leaf.size(), f) == leaf.size() &&
+            fread(icr.data(), 4, icr.size(), f) == icr.size() && fread(u.data(), 8, u.size(), f) == u.size();
+  fclose(f);
+  if (!ok) { fprintf(stderr, "writebin: short state file\n"); return 2; }
+  // the C-ABI takes the Fortran column-major lp(ld,8) / icoord(ld,2)
+  std::vector<int32_t> lp((size_t)8 * nbmax), ic((size_t)2 * nbmax);
+  for (int nb = 0; nb < nbmax; ++nb) {
+    for (int k = 0; k < 8; ++k) lp[(size_t)k * nbmax + nb] = lpr[(size_t)nb * 8 + k];
+    for (int d = 0; d < 2; ++d) ic[(size_t)d * nbmax + nb] = icr[(size_t)nb * 2 + d];
+  }
+  wx_params p{};
+  p.neq = neq; p.nlevs = nlevs; p.nbrootx = 1; p.nbrooty = 1; p.nblocks = cap;
+  p.cv = 1.5; p.gamma = 5. / 3.; p.cfl = 0.4; p.Tempsc = 1; p.Psc = 1; p.tsc = 1; p.tol = 0.05; p.tol2 = 1.0;
+  for (int l = 0; l < nlevs; ++l) { p.dx[l] = 1.0 / (1 << l); p.dy[l] = 1.0 / (1 << l); }
+  wx_ctx *c = nullptr;
+  if (wx_create(&p, &c)) { fprintf(stderr, "writebin: create failed\n"); return 2; }
+  char e[512];
+  if (wx_upload(c, u.data(), WX_LAYOUT_BLOCK, lp.data(), leaf.data(), ic.data(), nbmax, nbmax, nbleafs)) {
+    wx_last_error(e, 512);
+    fprintf(stderr, "writebin: upload: %s\n", e);
+    return 3;
+  }
+  const std::string pp = std::string(dir) + "/pointers000.001.bin", bp = std::string(dir) + "/blocks000.001.bin";
+  if (wx_write_bin(c, pp.c_str(), bp.c_str(), ltrue)) {
+    wx_last_error(e, 512);
+    fprintf(stderr, "writebin: %s\n", e);
+    return 3;
+  }
+  wx_destroy(c);
+  printf("writebin: %d blocks, %d leaves written\n", nbmax, nbleafs);
   return 0;
 }

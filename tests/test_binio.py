@@ -83,3 +83,40 @@ def test_file_names():
     assert binio.names("out", 7, 2) == (os.path.join("out", "BIN", "pointers002.007.bin"),
                                          os.path.join("out", "BIN", "blocks002.007.bin"))
     assert binio.names("out", 7, None)[0] == os.path.join("out", "BIN", "pointers007.bin")
+
+
+# --- the writer against the reference's own output (src/output.f90) ---
+import json  # noqa: E402
+import sys  # noqa: E402
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+import bin_inputs as B  # noqa: E402
+from wxoracle import Oracle, RefLib, ref_available  # noqa: E402
+
+GOLD_BIN = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_bin_digests.json")))
+
+
+@pytest.mark.parametrize("variant,seed,steps", B.BIN_CASES)
+def test_writer_matches_reference_output(variant, seed, steps, tmp_path):
+    """wx_write_bin (host build, ASan) of an oracle state against the
+    reference's own OUTBIN output of the same state, byte for byte: the
+    reference built from its sources in oracle/_ref where present, else the
+    digests it wrote (tests/golden/make_golden_bin.py).  Capacity and the
+    .true. encoding are the reference build's (4096 blocks, flang's 1)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    O = Oracle()
+    cfg, pp, m = B.state(O, variant, seed, steps)
+    B.write_state_file(str(tmp_path / "state.bin"), m, cfg)
+    ours = tmp_path / "ours"
+    ours.mkdir()
+    r = subprocess.run([os.path.join(HERE, "hostsim_main"), "0", "0", "writebin", str(tmp_path / "state.bin"),
+                        str(ours)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    op, ob = str(ours / "pointers000.001.bin"), str(ours / "blocks000.001.bin")
+    if ref_available(variant):
+        rp, rb = B.reference_files(RefLib(variant), pp, m, str(tmp_path / "ref"))
+        assert open(op, "rb").read() == open(rp, "rb").read()
+        assert open(ob, "rb").read() == open(rb, "rb").read()
+    g = GOLD_BIN[f"{variant}_{seed}_{steps}"]
+    assert [B.digest(op), os.path.getsize(op)] == g["pointers"]
+    assert [B.digest(ob), os.path.getsize(ob)] == g["blocks"]
