@@ -74,7 +74,7 @@ int launch_stageA(const wx_params *P, int, QueueRef q, int grid, int bs, int per
   return 0;
 }
 int launch_stageB(const wx_params *P, int, QueueRef q, int grid, int bs, int persist, const int2 *ops,
-                  const BlockRec *binfo, double *U, const double *Uh, DevScalars *S, hipStream_t st) {
+                  const BlockRec *binfo, double *U, double *Uh, DevScalars *S, hipStream_t st) {
   hipsim::launch(st, {P, q.hdr, q.rec, ops, binfo, U, Uh, S}, [=] { check_shape(q, grid, bs, persist); });
   return 0;
 }

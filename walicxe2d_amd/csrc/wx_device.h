@@ -118,7 +118,7 @@ int launch_build_queue(const QueueHdr *hdr, const int32_t *inst_off, const int32
 int launch_stageA(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const double *U,
                   double *Q, DevScalars *S, hipStream_t stream);
 int launch_stageB(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const int2 *ops,
-                  const BlockRec *binfo, double *U, const double *Q, DevScalars *S, hipStream_t stream);
+                  const BlockRec *binfo, double *U, double *Q, DevScalars *S, hipStream_t stream);
 int launch_fillI_cool(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const int2 *ops,
                       const BlockRec *binfo, const double *Uin, double *Uout, double *stat, int8_t *flag,
                       DevScalars *S, FlagOut fo, hipStream_t stream);

@@ -277,7 +277,7 @@ __global__ __launch_bounds__(BS) void k_stageB(const wx_params *__restrict__ P, 
                                                 const int2 *__restrict__ ops,
                                                 const BlockRec *__restrict__ binfo,
                                                 double *__restrict__ U, const double *__restrict__ Q,
-                                                DevScalars *__restrict__ Sa) {
+                                                double *__restrict__ Qring, DevScalars *__restrict__ Sa) {
   __shared__ double prim[NEQ * NC];
   __shared__ double F[NEQ * 272];
   __shared__ double G[NEQ * 272];
@@ -362,9 +362,14 @@ __global__ __launch_bounds__(BS) void k_stageB(const wx_params *__restrict__ P, 
   if (tid < 256)
 #pragma unroll
     for (int k = 0; k < NEQ; ++k) ub[k * NC + IDX(i, j)] = un[k];
+  // the ring of U^{n+1/2} goes to its own tile's ring in the U^{n+1/2}
+  // buffer (Qring is Q: this launch reads only interiors of Q), where the
+  // fill leaves layer 2 as the step's final value (src/tstep.f90:85 copies
+  // up's ring into u; boundaryI rewrites layer 1 only)
   if (ghost) {
+    double *rb = Qring + (size_t)b * NEQ * NC;
 #pragma unroll
-    for (int k = 0; k < NEQ; ++k) ub[k * NC + IDX(gi, gj)] = gh[k];
+    for (int k = 0; k < NEQ; ++k) rb[k * NC + IDX(gi, gj)] = gh[k];
   }
   if (err) atomicOr(&S->err, WX_EHLLC);
   __syncthreads();   // the next queue entry reuses the LDS tiles
@@ -608,37 +613,43 @@ __global__ __launch_bounds__(BS, (SUR ? 2 : 1024 / BS)) void k_fillI_cool(const 
   const bool imp = (L.flags & 1) != 0;
   const double vjet = p->otherb ? jet_velocity_at(p, S, time) : 0.0;
   const double *ib = Uin + (size_t)b * NEQ * NC;
-  // carried tile (interior U^{n+1}, ring of U^{n+1/2}); impose where listed
-  for (int c = tid; c < NC; c += blockDim.x) {
+  double *ob = Uout + (size_t)b * NEQ * NC;
+  // interior U^{n+1}; imposed where listed
+  for (int c = tid; c < 256; c += blockDim.x) {
+    const int i = c % 16 + 1, j = c / 16 + 1;
     double uu[NEQ];
 #pragma unroll
-    for (int k = 0; k < NEQ; ++k) uu[k] = ib[k * NC + c];
-    if (imp) {
-      const int i = c % NP - 1, j = c / NP - 1;
-      if (in_jet(p, L.level, L.icx, L.icy, i, j)) jet_state<NEQ>(p, vjet, uu);
-    }
+    for (int k = 0; k < NEQ; ++k) uu[k] = ib[k * NC + IDX(i, j)];
+    if (imp && in_jet(p, L.level, L.icx, L.icy, i, j)) jet_state<NEQ>(p, vjet, uu);
 #pragma unroll
-    for (int k = 0; k < NEQ; ++k) tile[k * NC + c] = uu[k];
+    for (int k = 0; k < NEQ; ++k) tile[k * NC + IDX(i, j)] = uu[k];
   }
-  __syncthreads();
-  // ghost layer 1 from the list (64 cells)
+  // ghost layer 1 (64 cells) from the list; a cell no entry writes keeps the
+  // U^{n+1/2} ring that stage B left in this tile's ring of Uout, imposed
+  // where listed
   if (tid < 64) {
     const int side = tid >> 4, pos = (tid & 15) + 1;
     const int2 op = ops[r * 8 + side * 2 + ((pos - 1) >> 3)];
+    int gi, gj;
+    ghost_coords(side, 1, pos, gi, gj);
+    double g[NEQ];
     if ((op.y & 15) != GK_NONE) {
-      double g[NEQ];
       ghost_gather<NEQ>(p, Uin, binfo, op, b, L.level, L.icx, L.icy, side, 1, pos, vjet, g);
-      int gi, gj;
-      ghost_coords(side, 1, pos, gi, gj);
+    } else {
 #pragma unroll
-      for (int k = 0; k < NEQ; ++k) tile[k * NC + IDX(gi, gj)] = g[k];
+      for (int k = 0; k < NEQ; ++k) g[k] = ob[k * NC + IDX(gi, gj)];
+      if (imp && in_jet(p, L.level, L.icx, L.icy, gi, gj)) jet_state<NEQ>(p, vjet, g);
     }
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) tile[k * NC + IDX(gi, gj)] = g[k];
   }
   // cooling over cells 0..n+1 (src/coolingh.f90:19-20, src/coolingdmc.f90:23-24)
   // except the four corner cells, which nothing reads: the interior (cell
   // c < 256) and the non-corner ghosts of layer 1 (c = 256 + the gather's
   // index).  With 256 threads a ghost is cooled by the thread that gathered
   // it, so the other waves start cooling the interior without waiting.
+  // Each cooled cell goes to Uout, and its primitives to the LDS tile for
+  // the epilogue's 5-point stencils.
   if (BS > 256) __syncthreads();
   const int mode = p->cooling;
   bool done = false;
@@ -650,36 +661,35 @@ __global__ __launch_bounds__(BS, (SUR ? 2 : 1024 / BS)) void k_fillI_cool(const 
       done = true;
     }
   }
-  if (!done && mode != 0) {
-    const double dts = S->dt * p->tsc;
-    for (int c = tid; c < 320; c += blockDim.x) {
-      int i, j;
-      cool_cell(c, i, j);
-      double uu[NEQ];
-#pragma unroll
-      for (int k = 0; k < NEQ; ++k) uu[k] = tile[k * NC + IDX(i, j)];
-      if (mode == 1) atomic_h<NEQ>(p, dts, uu);
-      else cool_dmc<NEQ>(p, dts, uu);
-#pragma unroll
-      for (int k = 0; k < NEQ; ++k) tile[k * NC + IDX(i, j)] = uu[k];
-    }
-  }
-  __syncthreads();
-  double *ob = Uout + (size_t)b * NEQ * NC;
-  for (int c = tid; c < NEQ * NC; c += blockDim.x) ob[c] = tile[c];
-  __syncthreads();
-  // primitives of the interior and the non-corner ghosts of layer 1 for the
-  // epilogue's 5-point stencils
+  const double dts = S->dt * p->tsc;
   for (int c = tid; c < 320; c += blockDim.x) {
     int i, j;
     cool_cell(c, i, j);
-    double uu[NEQ], q[NEQ], T;
+    double uu[NEQ];
 #pragma unroll
     for (int k = 0; k < NEQ; ++k) uu[k] = tile[k * NC + IDX(i, j)];
+    if (!done && mode == 1) atomic_h<NEQ>(p, dts, uu);
+    else if (!done && mode == 2) cool_dmc<NEQ>(p, dts, uu);
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k) ob[k * NC + IDX(i, j)] = uu[k];
+    double q[NEQ], T;
     uprim<NEQ>(p, q, uu, T);
 #pragma unroll
     for (int k = 0; k < NEQ; ++k) tile[k * NC + IDX(i, j)] = q[k];
   }
+  // ghost layer 2 keeps stage B's U^{n+1/2} ring (already in Uout); the
+  // source is imposed on its cells where listed
+  if (imp)
+    for (int g = tid; g < 64; g += blockDim.x) {
+      int gi, gj;
+      ghost_coords(g >> 4, 2, (g & 15) + 1, gi, gj);
+      if (in_jet(p, L.level, L.icx, L.icy, gi, gj)) {
+        double v[NEQ];
+        jet_state<NEQ>(p, vjet, v);
+#pragma unroll
+        for (int k = 0; k < NEQ; ++k) ob[k * NC + IDX(gi, gj)] = v[k];
+      }
+    }
   __syncthreads();
   block_stats<NEQ>(p, tile, L, stat, flag, fo);
   __syncthreads();   // the next queue entry reuses the LDS tile
@@ -984,9 +994,9 @@ int launch_stageA(const wx_params *P, int neq, QueueRef q, int grid, int bs, int
 }
 
 int launch_stageB(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const int2 *ops,
-                  const BlockRec *binfo, double *U, const double *Q, DevScalars *S, hipStream_t stream) {
+                  const BlockRec *binfo, double *U, double *Q, DevScalars *S, hipStream_t stream) {
   if (grid <= 0) return WX_OK;
-  WX_PERSIST_DISPATCH(persist, WX_BS_DISPATCH(bs, WX_DISPATCH(neq, hipLaunchKernelGGL((k_stageB<N, B, PS>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, U, Q, S))));
+  WX_PERSIST_DISPATCH(persist, WX_BS_DISPATCH(bs, WX_DISPATCH(neq, hipLaunchKernelGGL((k_stageB<N, B, PS>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, U, Q, Q, S))));
   return hip_status();
 }
 
