@@ -12,10 +12,9 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-12   # relative, per cell per step
 # Under COOLINGH the energy and the n_HI scalar are held to the same 1e-12:
-# the device's exp and pow are the host C library's bit for bit, its log is
-# the correctly rounded one (the library's differs from it on ~2e-4 of
-# arguments, by one ulp) and the jet speed's sin is the correctly rounded one
-# on the device (the library's differs on ~1.4e-3 of arguments).
+# the device's exp, pow and sin are the host C library's bit for bit and its
+# log is the correctly rounded one (the library's differs from it on ~2e-4
+# of arguments, by one ulp).
 TOL_HI = 1e-12
 
 

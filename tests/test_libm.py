@@ -1,10 +1,12 @@
 """The device's FP64 exp / pow / log / sin (walicxe2d_amd/csrc/wx_libm.h),
 compiled for the host, against the host C library on seeded random
-arguments: exp and pow equal to it bit for bit (the reference's rates and
-losses go through them, src/coolingh.f90:41-165), log and sin correctly
-rounded (checked against libquadmath where present); the C library's own log
-and sin are not correctly rounded on a small fraction of arguments, which the
-test bounds.  Also: the committed tables equal their generator's output."""
+arguments: exp, pow and sin equal to it bit for bit (the reference's rates
+and losses go through exp and pow, src/coolingh.f90:41-165, the jet speed
+through sin, JET/jet.f90:55), log correctly rounded (checked against
+libquadmath where present); the C library's own log is not correctly
+rounded on a small fraction of arguments, which the test bounds, and its sin
+is not on about 1.4e-3 of them (the device's sin follows the library there
+too).  Also: the committed tables equal their generator's output."""
 import os
 import shutil
 import subprocess
@@ -53,10 +55,12 @@ def test_libm_against_host_library(checker):
     r = {f[k]: int(f[k + 1]) for k in range(0, len(f), 2)}
     assert r["exp"] == 0, line
     assert r["pow"] == 0, line
-    # the C library's log / sin differ from the correctly rounded values on
-    # ~1e-4 / ~1.4e-3 of these arguments (by one ulp)
+    assert r["sin"] == 0, line
+    # the C library's log differs from the correctly rounded value on ~1e-4
+    # of these arguments (by one ulp)
     assert r["log"] <= 1e-3 * r["logtotal"], line
-    assert r["sin"] <= 5e-3 * n, line
     if has_quad:
         assert r["log_cr"] == 0, line
-        assert r["sin_cr"] == 0, line
+        # the library's sin (and so the device's) is not correctly rounded
+        # on a small fraction of arguments: the device does not round better
+        assert 0 < r["sin_cr"] <= 5e-3 * n, line

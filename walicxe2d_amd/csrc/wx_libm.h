@@ -13,10 +13,11 @@
 //               3 arguments in 10^5.  The host library's log is not correctly
 //               rounded on about 2 arguments in 10^4; there the two differ by
 //               one ulp.
-//   wsin        correctly rounded: double-double reduction by pi/2 and Taylor
-//               series, for |x| < 2^20 (ocml's sin beyond).  The host
-//               library's sin differs from the correctly rounded value on
-//               about 1.4 arguments in 10^3.
+//   wsin        the structure of the C library's sin (IBM Accurate
+//               Mathematical Library, glibc >= 2.28): table of sin/cos at
+//               i/128 (generated), the same series, reduction constants and
+//               fused multiply-adds; equal to the host library on every
+//               argument tested below |x| = 105414350 (beyond: ocml's sin).
 //
 // The reference computes its rates and losses with the Fortran intrinsics
 // exp, log, sin and the ** operator (src/coolingh.f90:41-165, JET/jet.f90:55,
@@ -244,78 +245,88 @@ WXM_FN double wlog(double x) {
 }
 
 // ---------------------------------------------------------------------------
-// Double-double arithmetic (value = hi + lo, |lo| <= ulp(hi)/2).
-struct dd {
-  double hi, lo;
-};
-WXM_FN dd dd_fast(double a, double b) {  // |a| >= |b|
-  const double s = a + b;
-  return dd{s, b - (s - a)};
-}
-WXM_FN dd dd_two_sum(double a, double b) {
-  const double s = a + b;
-  const double bb = s - a;
-  return dd{s, (a - (s - bb)) + (b - bb)};
-}
-WXM_FN dd dd_add(dd a, dd b) {
-  dd s = dd_two_sum(a.hi, b.hi);
-  const double e = s.lo + (a.lo + b.lo);
-  return dd_fast(s.hi, e);
-}
-WXM_FN dd dd_mul(dd a, dd b) {
-  const double p = a.hi * b.hi;
-  double e = fmad(a.hi, b.hi, -p);
-  e = fmad(a.hi, b.lo, fmad(a.lo, b.hi, e));
-  return dd_fast(p, e);
-}
-WXM_FN dd dd_tab(const uint64_t *t, int n) { return dd{asd(t[2 * n]), asd(t[2 * n + 1])}; }
+// sin: the structure of the C library's sin (glibc >= 2.28, the IBM Accurate
+// Mathematical Library's s_sin.c after the removal of its slow paths): a
+// table of sin and cos at i/128 (kSinCosTab) combined with short series of
+// the remainder, the argument reduced by a four-part pi/2 below |x| <
+// 105414350, a Taylor form near zero; the fused multiply-adds are those of
+// the library's FMA build.
+constexpr double kSinBig = 0x1.8p45;                  // 52776558133248: x + big rounds x to 1/128
+constexpr double kHp0 = 0x1.921fb54442d18p0, kHp1 = 0x1.1a62633145c07p-54;   // pi/2 = hp0 + hp1
+constexpr double kMp1 = 0x1.921fb58000000p0, kMp2 = -0x1.dde973c000000p-27;  // pi/2 in four parts
+constexpr double kPp3 = -0x1.cb3b398000000p-55, kPp4 = -0x1.d747f23e32ed7p-83;
+constexpr double kHpInv = 0x1.45f306dc9c883p-1, kToInt = 0x1.8p52;
+constexpr double kSn3 = -1.66666666666664880952546298448555E-01, kSn5 = 8.33333214285722277379541354343671E-03,
+                 kCs2 = 4.99999999999999999999950396842453E-01, kCs4 = -4.16666666666664434524222570944589E-02,
+                 kCs6 = 1.38888874007937613028114285595617E-03;
+constexpr double kS1 = -0x1.5555555555555p-3, kS2 = 0x1.1111111110ECEp-7, kS3 = -0.00019841269834414642,
+                 kS4 = 2.755729806860771e-06, kS5 = -2.5022014848318398e-08;
 
-// ---------------------------------------------------------------------------
-// sin: x = n pi/2 + a, |a| <= pi/4, with pi/2 to 3 doubles; sin or cos of a
-// by the Taylor series in double-double to a^31/31!.
-WXM_FN double sin_dd(double x) {
-  const double ax = __builtin_fabs(x);
-  if (ax < 0x1p-26) return x;
-  const double p1 = 0x1.921fb54442d18p0, p2 = 0x1.1a62633145c07p-54, p3 = -0x1.f1976b7ed8fbcp-110;
-  const double n = __builtin_rint(ax * 0x1.45f306dc9c883p-1);
-  // a = ax - n p1 - n p2 - n p3, each product split exactly
-  const double q1 = n * p1, e1 = fmad(n, p1, -q1);
-  dd a = dd_two_sum(ax - q1, -e1);  // ax - q1 is exact (Sterbenz) or ax itself (n = 0)
-  const double q2 = n * p2, e2 = fmad(n, p2, -q2);
-  a = dd_add(a, dd{-q2, -e2});
-  a = dd_add(a, dd{-n * p3, 0.0});
-  const int quad = (int)((int64_t)n & 3);
-  const dd a2 = dd_mul(a, a);
-  dd acc;
-  if (quad & 1) {
-    // cos a = sum (-1)^m a^(2m)/(2m)!
-    acc = dd_tab(kInvFact, 30);
-    for (int m = 14; m >= 0; --m) {
-      const dd c = dd_tab(kInvFact, 2 * m);
-      acc = dd_add(c, dd_mul(acc, dd{-a2.hi, -a2.lo}));
-    }
-  } else {
-    acc = dd_tab(kInvFact, 31);
-    for (int m = 14; m >= 0; --m) {
-      const dd c = dd_tab(kInvFact, 2 * m + 1);
-      acc = dd_add(c, dd_mul(acc, dd{-a2.hi, -a2.lo}));
-    }
-    acc = dd_mul(acc, a);
-  }
-  double r = acc.hi + acc.lo;
-  if (quad & 2) r = -r;
-  return x < 0.0 ? -r : r;
+// (a + da) - (a + da)^3/3! + ... near zero (|a| < 0.126)
+WXM_FN double sin_taylor(double xx, double a, double da) {
+  const double p = fmad(fmad(fmad(fmad(kS5, xx, kS4), xx, kS3), xx, kS2), xx, kS1);
+  const double t = fmad(fmad(p, a, -(0.5 * da)), xx, da);
+  return a + t;
+}
+
+// cos(x + dx) from the table entry nearest x
+WXM_FN double sin_do_cos(double x, double dx) {
+  if (x < 0) dx = -dx;
+  const double ux = kSinBig + __builtin_fabs(x);
+  x = __builtin_fabs(x) - (ux - kSinBig) + dx;
+  const double xx = x * x;
+  const double s = fmad(x * xx, fmad(xx, kSn5, kSn3), x);
+  const double c = xx * fmad(xx, fmad(xx, kCs6, kCs4), kCs2);
+  const int k = (int)((uint32_t)asu(ux) << 2);
+  const double sn = asd(kSinCosTab[k]), ssn = asd(kSinCosTab[k + 1]);
+  const double cs = asd(kSinCosTab[k + 2]), ccs = asd(kSinCosTab[k + 3]);
+  const double cor = fmad(-sn, s, fmad(-cs, c, fmad(-s, ssn, ccs)));
+  return cs + cor;
+}
+
+// sin(x + dx) from the table entry nearest x (Taylor form near zero)
+WXM_FN double sin_do_sin(double x, double dx) {
+  const double xold = x;
+  if (__builtin_fabs(x) < 0.126) return sin_taylor(x * x, x, dx);
+  if (x <= 0) dx = -dx;
+  const double ux = kSinBig + __builtin_fabs(x);
+  x = __builtin_fabs(x) - (ux - kSinBig);
+  const double xx = x * x;
+  const double s = x + fmad(x * xx, fmad(xx, kSn5, kSn3), dx);
+  const double c = fmad(x, dx, xx * fmad(xx, fmad(xx, kCs6, kCs4), kCs2));
+  const int k = (int)((uint32_t)asu(ux) << 2);
+  const double sn = asd(kSinCosTab[k]), ssn = asd(kSinCosTab[k + 1]);
+  const double cs = asd(kSinCosTab[k + 2]), ccs = asd(kSinCosTab[k + 3]);
+  const double cor = fmad(cs, s, fmad(-sn, c, fmad(s, ccs, ssn)));
+  return __builtin_copysign(sn + cor, xold);
 }
 
 WXM_FN double wsin(double x) {
-  if (!(__builtin_fabs(x) < 0x1p20)) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return ::sin(x);
-#else
-    return std::sin(x);
-#endif
+  const uint32_t k = (uint32_t)(asu(x) >> 32) & 0x7fffffffu;
+  if (k < 0x3e500000u) return x;                        // |x| < 2^-26
+  if (k < 0x3feb6000u) return sin_do_sin(x, 0.0);      // |x| < 0.855469
+  if (k < 0x400368fdu) {                                // |x| < 2.426265
+    const double t = kHp0 - __builtin_fabs(x);
+    return __builtin_copysign(sin_do_cos(t, kHp1), x);
   }
-  return sin_dd(x);
+  if (k < 0x419921fbu) {                                // |x| < 105414350
+    const double t = fmad(x, kHpInv, kToInt);
+    const double xn = t - kToInt;
+    const int n = (int)(asu(t) & 3);
+    const double y = fmad(-xn, kMp2, fmad(-xn, kMp1, x));
+    const double t2 = fmad(-xn, kPp3, y);
+    double db = fmad(-xn, kPp3, y - t2);
+    const double b = fmad(-xn, kPp4, t2);
+    db = db + fmad(-xn, kPp4, t2 - b);
+    const double r = (n & 1) ? sin_do_cos(b, db) : sin_do_sin(b, db);
+    return (n & 2) ? -r : r;
+  }
+  // beyond (not reached by the reference's jet speed): the device's own sin
+#if defined(__HIP_DEVICE_COMPILE__)
+  return ::sin(x);
+#else
+  return std::sin(x);
+#endif
 }
 
 }  // namespace libm
