@@ -187,6 +187,38 @@ def test_surrogate_jet_density_error():
     assert err <= 1e-3, err
 
 
+@pytest.mark.skipif(not os.path.exists(MODEL), reason="no trained model committed yet")
+def test_surrogate_full_size_c5_variant():
+    """The learned-cooling workload at its full size: a jet variant of the
+    surrogate's data set (64x16 roots, 4 levels, seed 777+100) with the
+    surrogate in the cooling pass against the exact routine over 1500 steps:
+    relative L1 density error <= 1e-3 after projection to the finest grid
+    (on the full jet the error grows past 1e-3 after about 2000 steps, as a
+    1e-3 change of the cooling rate does)."""
+    from walicxe2d_amd.l1 import l1_density
+    model = S.SurrogateModel.load(MODEL)
+    cfg = P.config_c5(100, nblocks=12000)
+    runs = []
+    for use in (False, True):
+        sim = Simulation(cfg)
+        sim.basegrid()
+        for _ in range(4):
+            sim.admesh()
+        if use:
+            sim.set_surrogate(model)
+        sim.step(1500)
+        u = sim.download()
+        lp, leaf, ic, _, _ = sim.tree()
+        cells, fb = sim.sur_counters()
+        runs.append((u, lp, leaf, ic, cells, fb))
+        sim.close()
+    err = l1_density(cfg, runs[1][:4], runs[0][:4])
+    cells, fb = runs[1][4], runs[1][5]
+    print(f"surrogate full-size jet variant: L1 density error {err:.3e}, fallback fraction {fb / max(cells, 1):.4f}")
+    assert cells > 0
+    assert err <= 1e-3, err
+
+
 def test_surrogate_off_is_the_exact_path():
     """Setting a model and removing it again leaves the plain path bit-exact."""
     cfg = P.config_c3(nbrootx=16, nbrooty=4, nblocks=4000)
