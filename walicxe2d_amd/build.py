@@ -8,6 +8,7 @@ order; packed FP32 instructions are disabled for every HIP translation unit.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -43,15 +44,18 @@ def _run(cmd):
 def build(verbose: bool = True) -> str:
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    jobs = []
     for name in KERNEL_SOURCES:
-        k_obj = os.path.join(objdir, name + ".o")
-        _run([HIPCC, *COMMON, *DEVICE, "-c", os.path.join(CSRC, name), "-o", k_obj])
-        objs.append(k_obj)
+        jobs.append(([HIPCC, *COMMON, *DEVICE, "-c", os.path.join(CSRC, name)], os.path.join(objdir, name + ".o")))
     for name in HOST_SOURCES:
-        h_obj = os.path.join(objdir, name + ".o")
-        _run([HIPCC, *COMMON, *DEVICE, "-x", "hip", "-c", os.path.join(CSRC, name), "-o", h_obj])
-        objs.append(h_obj)
+        jobs.append(([HIPCC, *COMMON, *DEVICE, "-x", "hip", "-c", os.path.join(CSRC, name)],
+                     os.path.join(objdir, name + ".o")))
+    # one compiler process per translation unit, a few at a time
+    workers = max(1, min(len(jobs), 8, os.cpu_count() or 1))
+    with concurrent.futures.ThreadPoolExecutor(workers) as pool:
+        for f in [pool.submit(_run, cmd + ["-o", obj]) for cmd, obj in jobs]:
+            f.result()
+    objs = [obj for _, obj in jobs]
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp, *objs])
     os.replace(tmp, LIB)

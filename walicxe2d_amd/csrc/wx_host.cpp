@@ -110,6 +110,9 @@ int upload_leaves(wx_ctx *c) {
       k = e;
     }
   }
+  // unknown until the copies are launched: a failure on the way leaves the
+  // shadow empty, so the next upload sends every record
+  dev.clear();
   if ((rc = stage_begin(c, al256(512 + bl) + runs.size() * 256 + al256(nchanged * sizeof(BlockRec))))) return rc;
   char *h = stage_take(c, 512 + bl);
   if (!h) return fail(WX_EARG, "staging %zu bytes", 512 + bl);
@@ -127,8 +130,9 @@ int upload_leaves(wx_ctx *c) {
     memcpy(hb, bi + r.first, bytes);
     if ((rc = stage_push(c, c->tBinfo + r.first, hb, bytes))) return rc;
   }
+  if ((rc = stage_end(c))) return rc;
   dev.assign(bi, bi + nb);
-  return stage_end(c);
+  return WX_OK;
 }
 
 // Second half: the ops of both fills (fill I's only where they differ from
