@@ -104,8 +104,25 @@ int wx_get_tree(wx_ctx *ctx, int32_t *lp, int32_t *leaf, int32_t *icoord, int32_
 /* Ghost-fill list from the host's locatebounds (src/locatebounds.f90:29-540):
  * innerbounds(ld,3) column-major, ninternal rows (codes -4..12, MPIP forms). */
 int wx_set_bounds(wx_ctx *ctx, const int32_t *innerbounds, int32_t ld, int32_t ninternal);
-/* Same list built by the library's own locatebounds from its tree. */
+/* Same list built by the library's own locatebounds from its tree (on the
+ * device unless wx_set_builder selected the host; *ninternal then waits for
+ * the device). */
 int wx_locatebounds(wx_ctx *ctx, int32_t *ninternal);
+/* Where the ghost-gather ops of the library's own list are built after a
+ * mesh change (src/locatebounds.f90:29-540 and the list order of
+ * src/boundaryII.f90:35-514, src/boundaryI.f90:34-396): 1 on the device,
+ * from the tree rows changed since the last build, beside the dt reduction
+ * and stage A; 0 on the host (the list itself, compiled, then uploaded);
+ * 2 on the device with the scan over block IDs spread over many workgroups
+ * at any mesh size (the form it takes above 2048 IDs); 3 (default) on the
+ * device from 3072 leaves up, else on the host.  All give identical
+ * tables.  A list set by wx_set_bounds is always compiled on the host. */
+int wx_set_builder(wx_ctx *ctx, int32_t on_device);
+/* The current gather tables, for checks: *nleaf leaf records (8 int32 each:
+ * id, level, icx, icy, flags, leaf position, instance, 0) and the ops of
+ * fills II and I (8 per leaf, 2 int32 each: source block, kind | flags).
+ * Copies only when *nleaf <= cap; synchronising. */
+int wx_get_tables(wx_ctx *ctx, int32_t *nleaf, int32_t *leaves, int32_t *ops2, int32_t *ops1, int32_t cap);
 
 /* src/timestep.f90:7-61: CFL (+ DMC cooling-time) timestep over leaf
  * interiors.  Synchronising. */
@@ -151,7 +168,8 @@ int wx_set_time(wx_ctx *ctx, double time);
  * 0 stage A, 1 stage B, 2 boundaryI+cooling, 3 dt reduction, 4 block
  * statistics, 5 refine/coarse ops; 6 host wall time running admesh on the
  * flags (after the wait for the step), 7 host wall time compiling and
- * uploading the ghost lists
+ * uploading the ghost lists, 8 the device table builder (tree rows and
+ * the builder's kernels)
  * (n entries); then, if enable >= 0, resets
  * the counters and switches recording on (1) or off (0). */
 int wx_profile(wx_ctx *ctx, int32_t enable, double *ms, int64_t *counts, int32_t n);

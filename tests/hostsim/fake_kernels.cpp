@@ -11,6 +11,7 @@
 #include <mutex>
 #include <vector>
 #include "../../walicxe2d_amd/csrc/wx_surrogate.h"
+#include "lb_cpu.h"
 
 // Flags by slot; with an ensemble split in groups, one table per group,
 // the group of an instance read from its jet[11] (a field the jet-less
@@ -153,4 +154,11 @@ int sur_counters(SurDev *, hipStream_t, int, int64_t *c, int64_t *f) {
   return 0;
 }
 int launch_calcprim(const wx_params *, int, int, const double *, double *, hipStream_t) { return 0; }
+int launch_build_tables(const wx_params &p, const TreeRow *T, int nbmax, int nleaf_host, LbInfo *info, LbPart *part,
+                        LbOut out, int, DevScalars *S, hipStream_t st) {
+  const wx_params pp = p;
+  hipsim::launch(st, {T, info, part, out.nlist, out.ops2, out.ops1, S},
+                 [=] { lb_cpu_build(pp, T, nbmax, nleaf_host, info, out, S); });
+  return 0;
+}
 }  // namespace wx

@@ -4,7 +4,10 @@
 // admesh given the same flags after every round.  Built with ASan/UBSan.
 // Usage: hostsim_main <rounds> <seed> [ens].  With "ens", three instances of
 // different root layouts, depths and capacities run as one wx_ens ensemble
-// and each instance's tree is compared with its own oracle mesh.
+// and each instance's tree is compared with its own oracle mesh.  The
+// single-mesh run cycles the table builder through its modes (host list,
+// device builder with either scan, choice by size) and checks the tables it
+// holds after every round against a host compilation of the same tree.
 // hostsim_main 0 0 writebin STATE DIR uploads the state in the file STATE
 // and writes its BIN pair into DIR with wx_write_bin (write_bin_of_state).
 // TEST SCAFFOLDING.
@@ -14,8 +17,11 @@
 #include <string>
 #include <vector>
 
+#include <cstring>
+
 #include "../../include/wx_capi.h"
 #include "../../oracle/wxoracle.h"
+#include "../../walicxe2d_amd/csrc/wx_runtime.h"
 #include "hip/hip_runtime.h"
 
 extern "C" void wxsim_set_flags(const signed char *f, int n);
@@ -25,6 +31,30 @@ extern "C" void wxsim_set_group_flags(int group, const signed char *f, int n);
 static int run_ensemble(int rounds, unsigned seed, int groups);
 static int bin_roundtrip(wx_ctx *c, const wx_params &p, const std::string &dir);
 static int write_bin_of_state(const char *state, const char *dir);
+
+// The context's current tables (wx_get_tables) against compile_leaves +
+// locatebounds + compile_ops on a copy of its tree.
+static int check_tables(wx_ctx *c, const wx_params &p, int r) {
+  int32_t n = 0, nbmax = 0, nbleafs = 0;
+  if (wx_get_tables(c, &n, nullptr, nullptr, nullptr, 0)) return 3;
+  std::vector<int32_t> rec((size_t)n * 8), o2((size_t)n * 16), o1((size_t)n * 16);
+  if (wx_get_tables(c, &n, rec.data(), o2.data(), o1.data(), n)) return 3;
+  if (wx_info(c, &nbmax, nullptr, nullptr)) return 3;
+  const int ld = nbmax > 0 ? nbmax : 1;
+  std::vector<int32_t> lp((size_t)8 * ld), leaf(ld), ic((size_t)2 * ld);
+  if (wx_get_tree(c, lp.data(), leaf.data(), ic.data(), ld, &nbmax, &nbleafs)) return 3;
+  wx::Mesh m;
+  m.init(p);
+  if (wx::set_tree(&m, lp.data(), leaf.data(), ic.data(), ld, nbmax, nbleafs, nullptr) || wx::refresh_tables(&m))
+    return 3;
+  if ((size_t)n != m.leaves.size()) { fprintf(stderr, "round %d: %d leaves vs %zu\n", r, n, m.leaves.size()); return 1; }
+  if (memcmp(rec.data(), m.leaves.data(), (size_t)n * 32) || memcmp(o2.data(), m.opsII.data(), (size_t)n * 64) ||
+      memcmp(o1.data(), m.opsI.data(), (size_t)n * 64)) {
+    fprintf(stderr, "round %d: tables differ from the host compilation\n", r);
+    return 1;
+  }
+  return 0;
+}
 
 int main(int argc, char **argv) {
   const int rounds = argc > 1 ? atoi(argv[1]) : 200;
@@ -62,6 +92,7 @@ int main(int argc, char **argv) {
       else if (x < pup + pdn) { f[nb] = 2; dn[nb] = 1; }
     }
     wxsim_set_flags(f.data(), cap + 1);
+    if (wx_set_builder(c, (r / 7) % 4)) { fprintf(stderr, "set_builder failed\n"); return 3; }
     double t, dt, z;
     if (wx_step(c, 1, 1, &t, &dt, &z)) {
       char b[512];
@@ -92,11 +123,13 @@ int main(int argc, char **argv) {
         return 1;
       }
     }
+    if (int rc = check_tables(c, p, r)) return rc;
     int32_t nint = 0;
     if (wx_locatebounds(c, &nint)) { fprintf(stderr, "locatebounds failed\n"); return 3; }
     if (nint != wxo_locatebounds((const wxo_params *)&p, &m)) { fprintf(stderr, "round %d: list length differs\n", r); return 1; }
   }
-  printf("hostsim: %d rounds, %d mesh ops, final nbmax %d leaves %d -- trees identical\n", rounds, total_ops, m.nbmax, m.nbleafs);
+  printf("hostsim: %d rounds, %d mesh ops, final nbmax %d leaves %d -- trees identical, tables identical\n", rounds,
+         total_ops, m.nbmax, m.nbleafs);
   if (argc > 4 && std::string(argv[3]) == "bin") {
     const int rc = bin_roundtrip(c, p, argv[4]);
     if (rc) return rc;

@@ -7,13 +7,18 @@
 // src/boundaryI.f90:87), imposition flags from the first and last list
 // position of each block.  Random meshes (seeded refinement rounds through
 // the product's admesh) for every combination of jet source, OUTFYB compat
-// mode and bottom/left boundary kind.  TEST SCAFFOLDING.
+// mode and bottom/left boundary kind.  The device table builder
+// (walicxe2d_amd/csrc/wx_lbuild.hip, run on the CPU by lb_cpu.h) is checked
+// against the host compilation on the same meshes, from a copy of the tree
+// kept up to date only through the rows the tree code marks as written (as
+// the runtime uploads them).  TEST SCAFFOLDING.
 #include <climits>
 #include <cstdio>
 #include <random>
 #include <vector>
 
 #include "../../walicxe2d_amd/csrc/wx_runtime.h"
+#include "lb_cpu.h"
 
 using namespace wx;
 
@@ -87,6 +92,58 @@ int restated(const Mesh &m, Tables &t) {
   return 0;
 }
 
+// The builder's copy of the tree brought up to date from the rows the tree
+// code marked (every row up to nbmax after a new tree), as upload_tree does.
+void update_copy(Mesh &m, std::vector<TreeRow> &dev) {
+  auto put = [&](int nb) {
+    TreeRow &r = dev[nb];
+    for (int k = 0; k < 8; ++k) r.lp[k] = m.lp[(size_t)nb * 8 + k];
+    r.leaf = m.leaf[nb];
+    r.icx = m.icoord[(size_t)nb * 2];
+    r.icy = m.icoord[(size_t)nb * 2 + 1];
+    r.pad = 0;
+  };
+  if (m.allDirty)
+    for (int nb = 1; nb <= m.nbmax; ++nb) put(nb);
+  else
+    for (int nb : m.dirtyRows) put(nb);
+  for (int nb : m.dirtyRows) m.rowDirty[nb] = 0;
+  m.dirtyRows.clear();
+  m.allDirty = false;
+}
+
+// Builder output against the host compilation of the same tree; 0 if equal.
+int check_builder(const Mesh &m, const std::vector<TreeRow> &dev, const char *tag) {
+  for (int nb = 1; nb <= m.nbmax; ++nb) {
+    bool same = dev[nb].leaf == m.leaf[nb] && dev[nb].icx == m.icoord[(size_t)nb * 2] &&
+                dev[nb].icy == m.icoord[(size_t)nb * 2 + 1];
+    for (int k = 0; k < 8; ++k) same = same && dev[nb].lp[k] == m.lp[(size_t)nb * 8 + k];
+    if (!same) { fprintf(stderr, "%s: tree row %d missed by the written-row marks\n", tag, nb); return 1; }
+  }
+  const int nleaf = (int)m.leaves.size(), cap = nleaf + 3;
+  std::vector<LbInfo> info((size_t)m.nbmax + 1);
+  std::vector<int2> o2((size_t)cap * 8, int2{-7, -7}), o1((size_t)cap * 8, int2{-7, -7});
+  int32_t nlist = -1;
+  DevScalars S{};
+  const LbOut out{&nlist, o2.data(), o1.data(), cap};
+  lb_cpu_build(m.p, dev.data(), m.nbmax, nleaf, info.data(), out, &S);
+  if (S.mesh_err) { fprintf(stderr, "%s: builder error %d\n", tag, S.mesh_err); return 1; }
+  if (nlist != m.ninternal) { fprintf(stderr, "%s: list length %d vs %d\n", tag, nlist, m.ninternal); return 1; }
+  for (int nb = 0; nb <= m.nbmax; ++nb)
+    if (info[nb].lidx != m.lidx[nb]) { fprintf(stderr, "%s: leaf position of %d differs\n", tag, nb); return 1; }
+  const bool two = lb_outfyb(m.p) != 0;
+  if (two == m.opsI_same) { fprintf(stderr, "%s: fill I table presence differs\n", tag); return 1; }
+  for (size_t k = 0; k < (size_t)nleaf * 8; ++k) {
+    const int2 d1 = two ? o1[k] : o2[k];
+    if (o2[k].x != m.opsII[k].x || o2[k].y != m.opsII[k].y || d1.x != m.opsI[k].x || d1.y != m.opsI[k].y) {
+      fprintf(stderr, "%s: builder segment %zu of leaf %d: II (%d,%d)/(%d,%d) I (%d,%d)/(%d,%d)\n", tag, k % 8,
+              m.leaves[k / 8].id, o2[k].x, o2[k].y, m.opsII[k].x, m.opsII[k].y, d1.x, d1.y, m.opsI[k].x, m.opsI[k].y);
+      return 1;
+    }
+  }
+  return 0;
+}
+
 }  // namespace
 
 int main() {
@@ -102,14 +159,26 @@ int main() {
       p.bc[0] = (variant & 8) ? 1 : 0; p.bc[1] = 0; p.bc[2] = (variant & 4) ? 1 : 0; p.bc[3] = 1;
       m.init(p);
       basegrid_tree(&m);
+      std::vector<TreeRow> dev((size_t)p.nblocks + 1, TreeRow{});
       std::mt19937 rng(1000 * variant + seed);
       for (int round = 0; round < 4; ++round) {
         if (refresh_tables(&m)) { fprintf(stderr, "refresh failed\n"); return 3; }
+        char tag[96];
+        snprintf(tag, sizeof tag, "variant %d seed %d round %d", variant, seed, round);
+        update_copy(m, dev);
+        if (check_builder(m, dev, tag)) return 1;
         std::vector<int8_t> fl(m.leaves.size(), 0);
         for (auto &f : fl) f = (rng() % 4 == 0) ? 1 : ((rng() % 3 == 0) ? 2 : 0);
         std::vector<MeshOp> ops;
         if (admesh_from_flags(&m, fl.data(), ops)) { fprintf(stderr, "admesh failed\n"); return 3; }
         if (refresh_tables(&m)) { fprintf(stderr, "refresh failed\n"); return 3; }
+        {   // the leaf records' imposition flags as compile_leaves sets them alone
+          std::vector<LeafRec> full = m.leaves;
+          compile_leaves(&m);
+          for (size_t l = 0; l < full.size(); ++l)
+            if (memcmp(&full[l], &m.leaves[l], sizeof(LeafRec))) { fprintf(stderr, "compile_leaves record %zu differs\n", l); return 1; }
+          m.leaves = full;
+        }
         Tables t;
         restated(m, t);
         for (size_t k = 0; k < t.o2.size(); ++k)
@@ -126,9 +195,12 @@ int main() {
         for (size_t k = 0; k < t.o2.size() && !differ; ++k) differ = t.o1[k].x != t.o2[k].x || t.o1[k].y != t.o2[k].y;
         if (differ != !m.opsI_same) { fprintf(stderr, "opsI_same flag wrong\n"); return 1; }
         outfyb_cases += differ;
+        update_copy(m, dev);
+        if (check_builder(m, dev, tag)) return 1;
         ++cases;
       }
     }
-  printf("tables_check: %d meshes (%d with fill I != fill II) -- tables identical\n", cases, outfyb_cases);
+  printf("tables_check: %d meshes (%d with fill I != fill II) -- tables identical, device builder identical\n", cases,
+         outfyb_cases);
   return 0;
 }

@@ -1,7 +1,9 @@
 """The product's host runtime (walicxe2d_amd/csrc/wx_host.cpp: admesh tree
-logic, table compilation, staging) built for the CPU against a host-only HIP
-stand-in with ASan/UBSan, driven with seeded refinement flags and compared
-with the oracle's admesh after every round (tests/hostsim)."""
+logic, table compilation, staging, the device table builder's uploads and
+launches) built for the CPU against a host-only HIP stand-in with
+ASan/UBSan, driven with seeded refinement flags and compared with the
+oracle's admesh after every round; the tables it holds (in each builder
+mode in turn) against a host compilation of its tree (tests/hostsim)."""
 import os
 import shutil
 import subprocess
@@ -18,7 +20,7 @@ def test_host_tree_matches_oracle_under_asan(seed):
     r = subprocess.run([os.path.join(HERE, "hostsim_main"), "250", str(seed)], capture_output=True, text=True,
                        timeout=600)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
-    assert "trees identical" in r.stdout
+    assert "trees identical, tables identical" in r.stdout
 
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not installed")

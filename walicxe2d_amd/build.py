@@ -30,7 +30,7 @@ DEVICE = [
 
 
 # device translation units
-KERNEL_SOURCES = ["wx_kernels.hip", "wx_surrogate.hip", "wx_noalloc.hip"]
+KERNEL_SOURCES = ["wx_kernels.hip", "wx_surrogate.hip", "wx_noalloc.hip", "wx_lbuild.hip"]
 # host-side translation units
 HOST_SOURCES = ["wx_mesh.cpp", "wx_runtime.cpp", "wx_host.cpp", "wx_ensemble.cpp", "wx_surrogate.cpp", "wx_bin.cpp",
                 "wx_loadbalance.cpp"]

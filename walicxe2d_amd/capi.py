@@ -34,6 +34,7 @@ EXPORTS = [
     "wx_shape", "wx_write_bin", "wx_read_bin", "wx_ens_shape", "wx_ens_write_bin",
     "wx_sur_pairs", "wx_sur_features", "wx_sur_infer", "wx_sur_grad", "wx_sur_train", "wx_set_surrogate",
     "wx_ens_set_surrogate", "wx_sur_counters", "wx_ens_sur_counters", "wx_lb_plan", "wx_sur_infer_noalloc",
+    "wx_set_builder", "wx_get_tables",
 ]
 
 _fp = ctypes.POINTER(ctypes.c_float)
@@ -46,7 +47,8 @@ class WxSurModel(ctypes.Structure):
                 ("out_mu", ctypes.c_float * 2), ("out_sig", ctypes.c_float * 2), ("lo", ctypes.c_float * 4),
                 ("hi", ctypes.c_float * 4), ("tol", ctypes.c_float), ("pad", ctypes.c_int32 * 3)]
 
-KERNELS = ["stageA", "stageB", "fillI_cool", "dtreduce", "blockstat", "meshops", "host_admesh", "host_tables"]
+KERNELS = ["stageA", "stageB", "fillI_cool", "dtreduce", "blockstat", "meshops", "host_admesh", "host_tables",
+           "builder"]
 
 
 class WxError(RuntimeError):
@@ -70,6 +72,8 @@ def load_library(path: str = LIBPATH) -> ctypes.CDLL:
     L.wx_get_tree.argtypes = [P, _ip, _ip, _ip, ctypes.c_int32, _ip, _ip]
     L.wx_set_bounds.argtypes = [P, _ip, ctypes.c_int32, ctypes.c_int32]
     L.wx_locatebounds.argtypes = [P, _ip]
+    L.wx_set_builder.argtypes = [P, ctypes.c_int32]
+    L.wx_get_tables.argtypes = [P, _ip, _ip, _ip, _ip, ctypes.c_int32]
     L.wx_timestep.argtypes = [P, _dp]
     L.wx_tstep.argtypes = [P, ctypes.c_double, ctypes.c_double]
     L.wx_criteria.argtypes = [P, _ip, _ip]
@@ -264,6 +268,25 @@ class Simulation:
         _check(self.L.wx_locatebounds(self.h, ctypes.byref(n)), "wx_locatebounds")
         return n.value
 
+    def set_builder(self, on_device):
+        """Build the gather ops on the device (True / 1) or on the host
+        (False / 0); 2: on the device with the many-workgroup scan at any
+        size; 3 (the default): on the device from 3072 leaves up."""
+        mode = on_device if isinstance(on_device, int) and not isinstance(on_device, bool) else (1 if on_device else 0)
+        _check(self.L.wx_set_builder(self.h, mode), "wx_set_builder")
+
+    def tables(self):
+        """The current gather tables: leaf records (nleaf, 8), ops of fills II
+        and I (nleaf, 8, 2) as int32 arrays."""
+        n = ctypes.c_int32()
+        _check(self.L.wx_get_tables(self.h, ctypes.byref(n), None, None, None, 0), "wx_get_tables")
+        m = n.value
+        rec = np.zeros((m, 8), dtype=np.int32)
+        o2 = np.zeros((m, 8, 2), dtype=np.int32)
+        o1 = np.zeros((m, 8, 2), dtype=np.int32)
+        _check(self.L.wx_get_tables(self.h, ctypes.byref(n), _i(rec), _i(o2), _i(o1), m), "wx_get_tables")
+        return rec, o2, o1
+
     def timestep(self) -> float:
         dt = np.zeros(1)
         _check(self.L.wx_timestep(self.h, _d(dt)), "wx_timestep")
@@ -292,9 +315,10 @@ class Simulation:
     def profile(self, enable: int = -1):
         """Per-kernel (ms, launches) since the last reset; enable 1/0 resets and
         switches event recording on/off, -1 only reads."""
-        ms = np.zeros(8)
-        n = np.zeros(8, dtype=np.int64)
-        _check(self.L.wx_profile(self.h, enable, _d(ms), n.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 8),
+        ms = np.zeros(len(KERNELS))
+        n = np.zeros(len(KERNELS), dtype=np.int64)
+        _check(self.L.wx_profile(self.h, enable, _d(ms), n.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                 len(KERNELS)),
                "wx_profile")
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(KERNELS)}
 
@@ -456,8 +480,9 @@ class Ensemble:
         return c.value, f.value
 
     def profile(self, enable: int = -1):
-        ms = np.zeros(8)
-        n = np.zeros(8, dtype=np.int64)
-        _check(self.L.wx_ens_profile(self.h, enable, _d(ms), n.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 8),
+        ms = np.zeros(len(KERNELS))
+        n = np.zeros(len(KERNELS), dtype=np.int64)
+        _check(self.L.wx_ens_profile(self.h, enable, _d(ms), n.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                     len(KERNELS)),
                "wx_ens_profile")
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(KERNELS)}

@@ -75,7 +75,37 @@ struct DevScalars {
   double vjet, vjet_time;   // jet source speed at vjet_time, valid when vjet_ok (set by k_dtreduce)
   int32_t err;
   int32_t vjet_ok;
-  int32_t pad[2];
+  int32_t mesh_err;         // the device table builder met a tree it cannot read (LBE_* code)
+  int32_t pad;
+};
+
+// One block of the AMR tree as the device table builder reads it
+// (src/globals.f90:13-29): lp(nb, 1..8) = level, father, first son, sibling
+// number, left, right, bottom, top neighbour; leaf(nb); icoord(nb, 1..2).
+struct TreeRow {
+  int32_t lp[8];
+  int32_t leaf;
+  int32_t icx, icy;
+  int32_t pad;
+};
+
+// Per block ID, written by the table builder: leaf position (-1: not a
+// leaf), position of its first entry in the ordered ghost list, and its
+// number of entries.
+struct LbInfo {
+  int32_t lidx, off, cnt, pad;
+};
+
+// Scan partials of the table builder: leaves and list entries of 256 IDs.
+struct LbPart {
+  int32_t leaves, entries;
+};
+
+// mesh_err codes of the table builder
+enum LbError : int32_t {
+  LBE_POINTER = 1,    // a neighbour or son pointer outside 1..nbmax
+  LBE_FAMILY = 2,     // a coarser neighbour's entry pair does not hold the block
+  LBE_COUNT = 3,      // the leaf count differs from the host's
 };
 
 // Mesh data ops for admesh (src/refine.f90, src/coarse.f90).
@@ -142,6 +172,27 @@ int launch_meshops(const wx_params *P, int neq, const MeshOp *ops, int nops, dou
 int launch_h2d(const CopyDesc *d, int n, hipStream_t stream);
 int launch_transpose(int neq, int cap, const double *src, double *dst, int to_fortran, int ld,
                      hipStream_t stream);
+
+// The OUTFYB compat mode of src/boundaryI.f90:87 is in effect: a bottom box
+// entry of fill I writes block e + 1 (e its list position) instead of its own.
+inline int lb_outfyb(const wx_params &p) { return (p.compat & 2) && p.bc[2] == 0; }
+
+// Where the device table builder writes: the list length, and the ops of
+// fills II and I (8 per leaf position, capacity leaf_cap leaves; fill I's
+// only in the OUTFYB mode, else the fill reads fill II's).
+struct LbOut {
+  int32_t *nlist;
+  int2 *ops2, *ops1;
+  int32_t leaf_cap;
+};
+// Builds the ghost ops of the tree T[0..nbmax] (wx_lbuild.hip): they equal
+// locatebounds + compile_ops on the host; the leaf positions are those of
+// compile_leaves.  info holds nbmax + 1 entries, part (nbmax + 256) / 256.
+// nleaf_host: the leaf count the host's tree holds (a different count sets
+// S->mesh_err).  multi_wg_scan: scan over many workgroups at any size (else
+// only above 2048 IDs).
+int launch_build_tables(const wx_params &p, const TreeRow *T, int nbmax, int nleaf_host, LbInfo *info, LbPart *part,
+                        LbOut out, int multi_wg_scan, DevScalars *S, hipStream_t stream);
 int launch_calcprim(const wx_params *P, int neq, int cap, const double *U, double *prim, hipStream_t stream);
 
 }  // namespace wx

@@ -51,6 +51,19 @@ struct wx_ctx : Mesh, Runtime {
   // cooling surrogate (coolingh only), used when use_sur
   DevBuf<SurDev> dSur;
   bool use_sur = false;
+  // device table builder: the tree rows on the device and the builder's
+  // per-ID scan results; used after a mesh change unless the list is
+  // external (wx_set_bounds) or the host builder is selected
+  DevBuf<TreeRow> dTree;
+  DevBuf<LbInfo> dLbInfo;
+  DevBuf<LbPart> dLbPart;
+  DevBuf<int32_t> dLbLen;          // the list length it found
+  std::vector<int32_t> treeIds;    // scratch: the rows of an upload
+  hipStream_t side = nullptr;      // its stream (default priority, like the context's)
+  hipEvent_t evTree = nullptr, evOps = nullptr;   // upload done; ops done
+  int dev_builder = 3;             // wx_set_builder's mode: 0 host, 1 device, 2 device (many-workgroup scan), 3 by size
+  bool ops_pending = false;        // launched; the context's stream has not waited for it yet
+  bool tables_on_device = false;   // the current tables came from it (the list length is on the device)
 
   double *cur_u() { return U[cur].ptr; }
   double *other_u() { return U[cur ^ 1].ptr; }
@@ -83,10 +96,63 @@ int arena_fit(wx_ctx *c) {
   return WX_OK;
 }
 
+// The leaf count from which the device builds the ops when the builder is
+// chosen by size: below it the host compiles the list while the device runs
+// the dt reduction and stage A, and that is the faster of the two (full
+// cooling jet, 1144 leaves: 101 us/step with the host list, 107 with the
+// device builder; point explosion, 5236 leaves: 202 and 192; 9939 leaves:
+// 357 and 311).
+constexpr size_t kDeviceBuilderLeaves = 3072;
+
+bool use_device_builder(const wx_ctx *c) {
+  if (c->dev_builder == 0 || c->bounds_external) return false;
+  return c->dev_builder != 3 || c->leaves.size() >= kDeviceBuilderLeaves;
+}
+
+// Staged copies of the tree rows written since the device copy was last
+// brought up to date (every row up to nbmax after a new tree), in runs of
+// consecutive IDs; the staging round must be open with room for
+// tree_bytes(c).
+size_t tree_bytes(const wx_ctx *c) {
+  const size_t n = c->allDirty ? (size_t)c->nbmax : c->dirtyRows.size();
+  return al256(n * sizeof(TreeRow)) + n * 256;
+}
+
+int stage_tree(wx_ctx *c) {
+  std::vector<int32_t> &ids = c->treeIds;
+  ids.clear();
+  if (c->allDirty)
+    for (int nb = 1; nb <= c->nbmax; ++nb) ids.push_back(nb);
+  else
+    ids.assign(c->dirtyRows.begin(), c->dirtyRows.end());
+  std::sort(ids.begin(), ids.end());
+  int rc;
+  for (size_t k = 0; k < ids.size();) {
+    size_t e = k + 1;
+    while (e < ids.size() && ids[e] == ids[e - 1] + 1) ++e;
+    const size_t bytes = (e - k) * sizeof(TreeRow);
+    TreeRow *hr = (TreeRow *)stage_take(c, bytes);
+    if (!hr) return fail(WX_EARG, "staging %zu tree rows", e - k);
+    for (size_t q = k; q < e; ++q) {
+      const int nb = ids[q];
+      if (nb < 1 || nb > c->cap) return fail(WX_EARG, "tree row %d outside 1..%d", nb, c->cap);
+      TreeRow &r = hr[q - k];
+      memcpy(r.lp, &c->lp[(size_t)nb * 8], sizeof r.lp);
+      r.leaf = c->leaf[nb];
+      r.icx = c->icoord[(size_t)nb * 2];
+      r.icy = c->icoord[(size_t)nb * 2 + 1];
+      r.pad = 0;
+    }
+    if ((rc = stage_push(c, c->dTree.ptr + ids[k], (const char *)hr, bytes))) return rc;
+    k = e;
+  }
+  return WX_OK;
+}
+
 // First half of a table upload: the queue header, the leaf records and the
 // block records (what the block statistics, the dt reduction and stage A
-// read).
-int upload_leaves(wx_ctx *c) {
+// read), and for the device builder the changed tree rows.
+int upload_leaves(wx_ctx *c, bool tree) {
   int rc = arena_fit(c);
   if (rc) return rc;
   const size_t nleaf = c->leaves.size();
@@ -113,7 +179,8 @@ int upload_leaves(wx_ctx *c) {
   // unknown until the copies are launched: a failure on the way leaves the
   // shadow empty, so the next upload sends every record
   dev.clear();
-  if ((rc = stage_begin(c, al256(512 + bl) + runs.size() * 256 + al256(nchanged * sizeof(BlockRec))))) return rc;
+  const size_t tb = tree ? tree_bytes(c) : 0;
+  if ((rc = stage_begin(c, al256(512 + bl) + runs.size() * 256 + al256(nchanged * sizeof(BlockRec)) + tb))) return rc;
   char *h = stage_take(c, 512 + bl);
   if (!h) return fail(WX_EARG, "staging %zu bytes", 512 + bl);
   const QueueHdr qh{(int32_t)nleaf, 1, {0, 0}};
@@ -130,8 +197,14 @@ int upload_leaves(wx_ctx *c) {
     memcpy(hb, bi + r.first, bytes);
     if ((rc = stage_push(c, c->tBinfo + r.first, hb, bytes))) return rc;
   }
+  if (tree && (rc = stage_tree(c))) return rc;
   if ((rc = stage_end(c))) return rc;
   dev.assign(bi, bi + nb);
+  if (tree) {   // the device copy of the tree is current once its copies are launched
+    for (int id : c->dirtyRows) c->rowDirty[id] = 0;
+    c->dirtyRows.clear();
+    c->allDirty = false;
+  }
   return WX_OK;
 }
 
@@ -162,16 +235,43 @@ int upload_ops(wx_ctx *c) {
 
 // After a mesh change the tables are rebuilt in two halves, so that a step
 // can launch the statistics, the dt reduction and stage A (which need only
-// the leaf records) before the ghost list is located and compiled.
+// the leaf records) before the ghost ops are ready.  With the device
+// builder the first half also launches the builder, on its own stream
+// after the upload; the second half makes the context's stream wait for
+// it.  Otherwise the host locates and compiles the list in the second half
+// while the device runs the first half's kernels.
 int tables_leaves(wx_ctx *c) {
   HostTimer ht(c, HID_TABLES);
   c->flags_host = false;   // leaf positions change
   int rc = compile_leaves(c);
-  return rc ? rc : upload_leaves(c);
+  const bool dev = use_device_builder(c);
+  if (!rc) rc = upload_leaves(c, dev);
+  if (rc || !dev) return rc;
+  HIPCHK(hipEventRecord(c->evTree, c->stream));
+  HIPCHK(hipStreamWaitEvent(c->side, c->evTree, 0));
+  const LbOut out{c->dLbLen.ptr, c->tOpsII, c->tOpsI, c->tLeafCap};
+  {
+    ProfScope ps(c, KID_BUILD, c->side);
+    rc = launch_build_tables(c->p, c->dTree.ptr, c->nbmax, c->nleaf(), c->dLbInfo.ptr, c->dLbPart.ptr, out,
+                             c->dev_builder == 2, c->dS.ptr, c->side);
+  }
+  if (rc) return fail(rc, "table builder launch failed");
+  HIPCHK(hipEventRecord(c->evOps, c->side));
+  c->ops_pending = true;
+  return WX_OK;
 }
 
 int tables_ops(wx_ctx *c) {
   HostTimer ht(c, HID_TABLES);
+  if (c->ops_pending) {
+    HIPCHK(hipStreamWaitEvent(c->stream, c->evOps, 0));
+    c->ops_pending = false;
+    c->opsI_same = !lb_outfyb(c->p);
+    c->tables_on_device = true;
+    c->tables_valid = true;
+    return WX_OK;
+  }
+  c->tables_on_device = false;
   if (!c->bounds_external) locatebounds(c);
   int rc = compile_ops(c);
   if (rc) return rc;
@@ -184,6 +284,19 @@ int ensure_tables(wx_ctx *c) {
   if (c->tables_valid) return WX_OK;
   int rc = tables_leaves(c);
   return rc ? rc : tables_ops(c);
+}
+
+// The list length: the host's, or the device builder's (with a wait).
+int list_length(wx_ctx *c, int32_t *n) {
+  if (!c->tables_on_device || !c->tables_valid) {
+    *n = c->ninternal;
+    return WX_OK;
+  }
+  int32_t h = 0;
+  HIPCHK(hipMemcpyAsync(&h, c->dLbLen.ptr, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *n = h;
+  return WX_OK;
 }
 
 // Block statistics of the leaves that lack them (all, or the new leaves of
@@ -222,14 +335,26 @@ int ensure_stats(wx_ctx *c) {
   return rc ? rc : update_stats(c);
 }
 
-int sync_scalars(wx_ctx *c) {
-  HIPCHK(hipMemcpyAsync(c->hS, c->dS.ptr, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+// The device's error flags after a wait: a Riemann failure of the stages,
+// or a tree the table builder could not read.
+int check_scalars(wx_ctx *c) {
   if (c->hS->err) {
     int e = c->hS->err;
     return fail(WX_EHLLC, "Error in hllc (device flag %d): a Riemann problem had no valid star speed", e);
   }
+  if (c->hS->mesh_err) {
+    int e = c->hS->mesh_err;
+    return fail(WX_EARG, "device table builder: the tree is inconsistent (code %d: %s)", e,
+                e == LBE_POINTER ? "a pointer outside 1..nbmax" : e == LBE_FAMILY ? "a coarser neighbour's sons"
+                : e == LBE_COUNT ? "leaf count" : "leaf level");
+  }
   return WX_OK;
+}
+
+int sync_scalars(wx_ctx *c) {
+  HIPCHK(hipMemcpyAsync(c->hS, c->dS.ptr, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return check_scalars(c);
 }
 
 // tstep: stage A, then stage B and the fill (which need the ops tables).
@@ -324,11 +449,7 @@ int fetch_leaf_flags(wx_ctx *c) {
     if (rc) return fail(rc, "flag gather launch failed");
   }
   HIPCHK(hipStreamSynchronize(c->stream));
-  if (c->hS->err) {
-    int e = c->hS->err;
-    return fail(WX_EHLLC, "Error in hllc (device flag %d): a Riemann problem had no valid star speed", e);
-  }
-  return WX_OK;
+  return check_scalars(c);
 }
 
 int do_admesh(wx_ctx *c, int *changed) {
@@ -371,7 +492,21 @@ int wx_create(const wx_params *p, wx_ctx **out) {
   if (!rc) rc = c->dS.ensure(1);
   if (!rc) rc = c->dStat.ensure((size_t)(c->cap + 1) * 2);
   if (!rc) rc = c->dFlag.ensure((size_t)c->cap + 1);
+  if (!rc) rc = c->dTree.ensure((size_t)c->cap + 1);
+  if (!rc) rc = c->dLbInfo.ensure((size_t)c->cap + 1);
+  if (!rc) rc = c->dLbPart.ensure((size_t)(c->cap + 256) / 256);
+  if (!rc) rc = c->dLbLen.ensure(1);
   if (rc) { wx_destroy(c); return rc; }
+  if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) {
+    c->side = nullptr;
+    wx_destroy(c);
+    return fail(WX_EHIP, "builder stream");
+  }
+  if (hipEventCreateWithFlags(&c->evTree, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->evOps, hipEventDisableTiming) != hipSuccess) {
+    wx_destroy(c);
+    return fail(WX_EHIP, "builder events");
+  }
   for (int k = 0; k < 2; ++k) (void)hipMemsetAsync(c->U[k].ptr, 0, tiles * sizeof(double), c->stream);
   (void)hipMemcpyAsync(c->dP.ptr, p, sizeof(wx_params), hipMemcpyHostToDevice, c->stream);
   DevScalars z{};
@@ -396,9 +531,14 @@ int wx_create(const wx_params *p, wx_ctx **out) {
 int wx_destroy(wx_ctx *c) {
   if (!c) return WX_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->side) (void)hipStreamSynchronize(c->side);
   for (auto &b : c->U) b.release();
   c->dP.release(); c->dS.release(); c->dTab.release(); c->dStat.release(); c->dFlag.release();
-  c->dScratch.release(); c->dSur.release();
+  c->dScratch.release(); c->dSur.release(); c->dTree.release(); c->dLbInfo.release(); c->dLbPart.release();
+  c->dLbLen.release();
+  if (c->evTree) (void)hipEventDestroy(c->evTree);
+  if (c->evOps) (void)hipEventDestroy(c->evOps);
+  if (c->side) (void)hipStreamDestroy(c->side);
   c->Runtime::destroy();
   if (c->hS) (void)hipHostFree(c->hS);
   if (c->hFlag) (void)hipHostFree(c->hFlag);
@@ -498,7 +638,33 @@ int wx_locatebounds(wx_ctx *c, int32_t *ninternal) {
   c->tables_valid = false;
   int rc = ensure_tables(c);
   if (rc) return rc;
-  if (ninternal) *ninternal = c->ninternal;
+  if (ninternal && (rc = list_length(c, ninternal))) return rc;
+  return c->tables_on_device ? sync_scalars(c) : WX_OK;
+}
+
+int wx_set_builder(wx_ctx *c, int32_t on_device) {
+  if (!c || on_device < 0 || on_device > 3) return fail(WX_EARG, "bad argument");
+  if (c->ops_pending) return fail(WX_EARG, "wx_set_builder: a table build is in progress");
+  if (on_device != c->dev_builder) {
+    c->dev_builder = on_device;
+    c->tables_valid = false;   // the next step builds them the new way
+  }
+  return WX_OK;
+}
+
+int wx_get_tables(wx_ctx *c, int32_t *nleaf, int32_t *leaves, int32_t *ops2, int32_t *ops1, int32_t cap) {
+  if (!c || !nleaf) return fail(WX_EARG, "null argument");
+  int rc = ensure_tables(c);
+  if (rc) return rc;
+  const int n = c->nleaf();
+  *nleaf = n;
+  if (n > cap) return WX_OK;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if ((rc = sync_scalars(c))) return rc;
+  if (leaves) HIPCHK(hipMemcpy(leaves, c->tLeaves, (size_t)n * sizeof(LeafRec), hipMemcpyDeviceToHost));
+  if (ops2) HIPCHK(hipMemcpy(ops2, c->tOpsII, (size_t)n * 8 * sizeof(int2), hipMemcpyDeviceToHost));
+  if (ops1)
+    HIPCHK(hipMemcpy(ops1, c->opsI_same ? c->tOpsII : c->tOpsI, (size_t)n * 8 * sizeof(int2), hipMemcpyDeviceToHost));
   return WX_OK;
 }
 
@@ -685,7 +851,7 @@ int wx_info(wx_ctx *c, int32_t *nbmax, int32_t *nbleafs, int32_t *ninternal) {
   if (!c) return fail(WX_EARG, "null context");
   if (nbmax) *nbmax = c->nbmax;
   if (nbleafs) *nbleafs = c->nbleafs;
-  if (ninternal) *ninternal = c->ninternal;
+  if (ninternal) return list_length(c, ninternal);
   return WX_OK;
 }
 

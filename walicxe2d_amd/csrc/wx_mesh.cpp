@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "wx_lbuild.h"
 #include "wx_runtime.h"
 
 namespace wx {
@@ -39,6 +40,9 @@ void Mesh::init(const wx_params &pp) {
   icoord.assign((size_t)(cap + 1) * 2, 0);
   up.assign((size_t)cap + 1, 0);
   dn.assign((size_t)cap + 1, 0);
+  rowDirty.assign((size_t)cap + 1, 0);
+  dirtyRows.clear();
+  allDirty = true;
   nbmax = nbleafs = ninternal = 0;
   tables_valid = false;
   bounds_external = false;
@@ -98,8 +102,16 @@ void locatebounds(Mesh *c) {
 }
 
 // ---------------------------------------------------------------------
-// The leaf records (block-ID order, imposition flags cleared), the leaf
-// position of every block ID and the block records, from the tree.
+// The host tree through the accessor of the list functions (wx_lbuild.h).
+struct MeshTree {
+  const int32_t *lp_, *leaf_;
+  int lp(int nb, int k) const { return lp_[(size_t)nb * 8 + k]; }
+  int leaf(int nb) const { return leaf_[nb]; }
+};
+
+// The leaf records (block-ID order, with the jet imposition flag of the
+// list the tree gives), the leaf position of every block ID and the block
+// records, from the tree.
 int compile_leaves(Mesh *c) {
   const int cap = c->nbmax;   // IDs above nbmax hold no block
   c->lidx.assign((size_t)cap + 1, -1);
@@ -115,6 +127,8 @@ int compile_leaves(Mesh *c) {
   }
   c->leaves.resize((size_t)nleaf);
   LeafRec *rec = c->leaves.data();
+  const bool otherb = c->p.otherb != 0;
+  const MeshTree tree{lp, leaf};
   for (int nb = 1; nb <= cap; ++nb) {
     const int l = lidx[nb];
     if (l < 0) continue;
@@ -124,6 +138,9 @@ int compile_leaves(Mesh *c) {
     r.icx = bi[nb].icx;
     r.icy = bi[nb].icy;
     r.lidx = l;
+    // impose_user_bc runs on the block in the fills when it makes list
+    // entries (compile_ops sets the same flag from the list itself)
+    if (otherb) r.flags = lb::block_count(tree, cap, nb) > 0 ? 1 : 0;
     if (r.level < 1 || r.level > c->p.nlevs) return fail(WX_EARG, "block %d has level %d", nb, r.level);
     rec[l] = r;
   }
@@ -141,7 +158,7 @@ int compile_ops(Mesh *c) {
   const int nleaf = (int)c->leaves.size();
   const size_t nseg = (size_t)nleaf * 8;
   const bool otherb = c->p.otherb != 0;
-  const bool outfyb_bug = (c->p.compat & 2) && c->p.bc[2] == 0;
+  const bool outfyb_bug = lb_outfyb(c->p) != 0;
   // Fills I and II receive the same writes except for the OUTFYB entries of
   // boundaryI (src/boundaryI.f90:87: a bottom box entry writes block e + 1
   // instead of its own block): fill II is compiled from the list without
@@ -294,6 +311,7 @@ static void updatelpup(Mesh *c, int nb, int s1, int s2, int s3, int s4, std::vec
 
   c->leaf[nb] = 0;
   c->leaf[s1] = c->leaf[s2] = c->leaf[s3] = c->leaf[s4] = 1;
+  c->touch(nb);
   up[nb] = 0; dn[nb] = 0;
   dn[s1] = dn[s2] = dn[s3] = dn[s4] = 0;
   const int nx0 = c->icoord[(size_t)nb * 2] * 2, ny0 = c->icoord[(size_t)nb * 2 + 1] * 2;
@@ -307,6 +325,7 @@ static void updatelpup(Mesh *c, int nb, int s1, int s2, int s3, int s4, std::vec
 static void updatelpdown(Mesh *c, int dad, int s1, int s2, int s3, int s4) {
   c->leaf[dad] = 1;
   c->leaf[s1] = c->leaf[s2] = c->leaf[s3] = c->leaf[s4] = 0;
+  c->touch(dad); c->touch(s1); c->touch(s2); c->touch(s3); c->touch(s4);
   c->nbleafs -= 3;
   int nb1 = c->LP(dad, 5);
   if (nb1 != -1 && !c->leaf[nb1]) { nb1 = c->LP(c->LP(nb1, 3), 6); c->SETLP(nb1, 6, dad); c->SETLP(c->LP(nb1, 8), 6, dad); }
@@ -468,6 +487,7 @@ int basegrid_tree(Mesh *c) {
   const int nroots = nrx * nry;
   if (nrx < 1 || nry < 1) return fail(WX_EARG, "root layout %d x %d", nrx, nry);
   if (nroots > c->cap) return fail(WX_EBLOCKS, "capacity %d below root count %d", c->cap, nroots);
+  c->all_dirty();
   std::fill(c->lp.begin(), c->lp.end(), 0);
   std::fill(c->leaf.begin(), c->leaf.end(), 0);
   std::fill(c->icoord.begin(), c->icoord.end(), 0);
@@ -507,6 +527,7 @@ int set_tree(Mesh *c, const int32_t *lp, const int32_t *leaf, const int32_t *ico
     c->nbleafs = nbleafs;
     return WX_OK;
   }
+  c->all_dirty();
   std::fill(c->lp.begin(), c->lp.end(), 0);
   std::fill(c->leaf.begin(), c->leaf.end(), 0);
   std::fill(c->icoord.begin(), c->icoord.end(), 0);

@@ -149,17 +149,17 @@ static hipEvent_t take_event(Runtime *r) {
   return e;
 }
 
-ProfScope::ProfScope(Runtime *r_, int kid_) : r(r_), kid(kid_) {
+ProfScope::ProfScope(Runtime *r_, int kid_, hipStream_t s_) : r(r_), kid(kid_), s(s_ ? s_ : r_->stream) {
   if (r->prof && kid >= 0) {
     a = take_event(r);
-    (void)hipEventRecord(a, r->stream);
+    (void)hipEventRecord(a, s);
   }
 }
 
 ProfScope::~ProfScope() {
   if (r->prof && a) {
     hipEvent_t b = take_event(r);
-    (void)hipEventRecord(b, r->stream);
+    (void)hipEventRecord(b, s);
     r->marks.push_back(Runtime::Mark{kid, a, b});
   }
 }

@@ -87,8 +87,27 @@ struct Mesh {
   std::vector<int> admUp, admDown;
   std::vector<std::vector<int>> admByLevel, admCand;
 
+  // rows of lp / leaf / icoord written since the device copy of the tree
+  // was last brought up to date (all of them while allDirty)
+  std::vector<int32_t> dirtyRows;
+  std::vector<uint8_t> rowDirty;
+  bool allDirty = true;
+
   int LP(int nb, int k) const { return lp[(size_t)nb * 8 + k - 1]; }
-  void SETLP(int nb, int k, int v) { lp[(size_t)nb * 8 + k - 1] = v; }
+  void touch(int nb) {
+    if (allDirty || rowDirty[nb]) return;
+    rowDirty[nb] = 1;
+    dirtyRows.push_back(nb);
+  }
+  void SETLP(int nb, int k, int v) {
+    lp[(size_t)nb * 8 + k - 1] = v;
+    touch(nb);
+  }
+  void all_dirty() {
+    for (int nb : dirtyRows) rowDirty[nb] = 0;
+    dirtyRows.clear();
+    allDirty = true;
+  }
   void init(const wx_params &pp);
 };
 
@@ -117,7 +136,7 @@ int get_tree(const Mesh *m, int32_t *lp, int32_t *leaf, int32_t *icoord, int32_t
 // ---------------------------------------------------------------------
 // Kernel IDs for the profiling counters; slots 6 and 7 are host wall time.
 enum { KID_STAGEA = 0, KID_STAGEB = 1, KID_FILLI = 2, KID_DTRED = 3, KID_STAT = 4, KID_MESH = 5,
-       HID_ADMESH = 6, HID_TABLES = 7, NPROF = 8 };
+       HID_ADMESH = 6, HID_TABLES = 7, KID_BUILD = 8, NPROF = 9 };
 
 struct Runtime {
   hipStream_t stream = nullptr;
@@ -167,11 +186,14 @@ int stage_end(Runtime *r);
 void prof_collect(Runtime *r);
 int prof_report(Runtime *r, int32_t enable, double *ms, int64_t *counts, int32_t n);
 
+// Events around the launches of its scope on stream s (the runtime's own
+// stream when s is null).
 struct ProfScope {
   Runtime *r;
   int kid;
+  hipStream_t s;
   hipEvent_t a = nullptr;
-  ProfScope(Runtime *r_, int kid_);
+  ProfScope(Runtime *r_, int kid_, hipStream_t s_ = nullptr);
   ~ProfScope();
 };
 
