@@ -55,11 +55,13 @@ class WxError(RuntimeError):
     pass
 
 
-def load_library(path: str = LIBPATH) -> ctypes.CDLL:
-    """Loads libwxhip.so; raises if it has not been built."""
+def load_library(path: str | None = None) -> ctypes.CDLL:
+    """Loads libwxhip.so (the module's LIBPATH unless a path is given); raises
+    if it has not been built."""
     global _lib
     if _lib is not None:
         return _lib
+    path = path or LIBPATH
     if not os.path.exists(path):
         raise WxError(f"{path} is missing: build it with `python -m walicxe2d_amd.build` "
                       "(the hot path has no CPU fallback)")

@@ -162,8 +162,10 @@ WXM_FN double pow_log(uint64_t ix, double *tail) {
   const double hi = t2 + ar2;
   const double lo3 = fmad(ar, r, -ar2);
   const double lo4 = t2 - hi + ar2;
-  const double p = ar3 * fmad(ar2, fmad(ar2, fmad(r, kA6, kA5), fmad(r, kA4, kA3)), fmad(r, kA2, kA1));
-  const double lo = lo1 + lo2 + lo3 + lo4 + p;
+  // the library's p = ar3 * (...) has one use, the last addition of lo, and
+  // is fused into it (one argument in 10^7 or fewer rounds differently)
+  const double lo = fmad(ar3, fmad(ar2, fmad(ar2, fmad(r, kA6, kA5), fmad(r, kA4, kA3)), fmad(r, kA2, kA1)),
+                         lo1 + lo2 + lo3 + lo4);
   const double y = hi + lo;
   *tail = hi - y + lo;
   return y;
