@@ -196,8 +196,17 @@ def roofline(prof, zones, neq, workload, peak=HBM_PEAK_GBS):
             bpz = json.load(fh).get("per_zone_update", {}).get(f"{dom}:{workload}")
         if bpz is not None:
             traffic = round(bpz * zones / max(launches, 1))
+    # the streaming-copy bandwidth measured on this GPU (walicxe2d_amd/copybw.py),
+    # the ceiling a kernel that reads and writes HBM can reach in practice
+    copy = None
+    cfile = os.path.join(ROOT, "profiles", "copy_ceiling.json")
+    if os.path.exists(cfile):
+        with open(cfile) as fh:
+            copy = json.load(fh).get("GBps_read_plus_write")
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": peak, "unit": "GB/s",
             "frac": round(achieved / peak, 4), "traffic": traffic,
+            "copy_ceiling": round(copy, 1) if copy else None,
+            "frac_of_copy_ceiling": round(achieved / copy, 4) if copy else None,
             "kernel": dom, "kernel_ms_per_launch": round(ms / max(launches, 1), 5),
             "alg_bytes_per_launch": round(alg_bytes / max(launches, 1)),
             "kernels_ms": {k: round(v[0], 3) for k, v in prof.items() if v[1]}}

@@ -1,8 +1,9 @@
 """Text summaries of rocprofv3 result databases (the `*_results.db` files
 that `rocprofv3 -d DIR -o NAME` writes).
 
-    python -m walicxe2d_amd.profsum kernels DB
-        per-kernel calls, total / average duration, share, VGPRs, LDS, scratch
+    python -m walicxe2d_amd.profsum kernels DB [--last N]
+        per-kernel calls, total / average duration, share, VGPRs, LDS,
+        scratch; with --last, each kernel's average over its last N launches
     python -m walicxe2d_amd.profsum valu DB [DB ...]
         per-wave SQ counters of `--pmc` runs of one command and the VALU
         issue share of each kernel
@@ -32,6 +33,20 @@ HOT = ("k_stageA", "k_stageB", "k_fillI_cool", "k_ens_stageA", "k_ens_stageB", "
 def short(name: str) -> str:
     m = re.search(r"wx::(k_\w+(?:<[^>]*>)?)", name)
     return m.group(1) if m else name[:48]
+
+
+def kernels_last(db: str, last: int) -> str:
+    """Average duration of the last `last` launches of each kernel (the
+    window a run times with HIP events at its end)."""
+    cur = sqlite3.connect(db).cursor()
+    rows = collections.defaultdict(list)
+    for name, start, end in cur.execute("select name, start, end from kernels order by start"):
+        rows[name].append(end - start)
+    out = [f"| kernel | launches averaged (the last) | average (us) |", "|---|---|---|"]
+    for n, d in sorted(rows.items(), key=lambda kv: -sum(kv[1][-last:])):
+        w = d[-last:]
+        out.append(f"| `{short(n)}` | {len(w)} | {sum(w) / len(w) / 1e3:.3f} |")
+    return "\n".join(out)
 
 
 def kernels(db: str) -> str:
@@ -123,6 +138,7 @@ def main(argv=None):
     sub = ap.add_subparsers(dest="cmd", required=True)
     k = sub.add_parser("kernels")
     k.add_argument("db")
+    k.add_argument("--last", type=int, default=0, help="also the average of each kernel's last N launches")
     v = sub.add_parser("valu")
     v.add_argument("dbs", nargs="+")
     t = sub.add_parser("traffic")
@@ -134,6 +150,9 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.cmd == "kernels":
         print(kernels(a.db))
+        if a.last:
+            print()
+            print(kernels_last(a.db, a.last))
     elif a.cmd == "valu":
         print(valu(a.dbs))
     else:
