@@ -53,6 +53,12 @@ L1_BOUND = 1e-3   # relative L1 density error the batched path is held to
 #   fillI_cool   read U^{n+1}, write the cooled U^{n+1}          2*neq*8
 # (the whole step's compulsory traffic is 5*neq*8: stages A and B).
 ALG_PASSES = {"stageA": 2, "stageB": 3, "fillI_cool": 2}
+_T0 = time.perf_counter()
+
+
+def _note(what: str):
+    """A progress line on stderr (stdout carries only the result line)."""
+    print(f"bench: {what} ({time.perf_counter() - _T0:.0f} s)", file=sys.stderr, flush=True)
 
 
 def _cpu_share() -> int:
@@ -365,6 +371,7 @@ def main():
     pv, pms, pleaf, proof = None, None, None, None
     if args.path in ("both", "plain"):
         pv, pms, _, _, _, pleaf = run_plain(cfg, args.steps, args.warmup)
+        _note(f"plain path: {pv:.3e} zone-updates/s")
         # the same run again with kernel events, for the roofline
         _, _, pprof, pzones, _, _ = run_plain(cfg, args.steps, args.warmup, events=True)
         proof = roofline(pprof, pzones, neq, args.workload)
@@ -396,6 +403,7 @@ def main():
         # runs of the same variants (so its L1 error is zero by construction)
         bopts = dict(groups=args.groups, tune=not args.no_tune, graph=not args.eager)
         bx = run_batched(cfgs, args.steps, args.warmup, None, nstates=args.batch, **bopts)
+        _note(f"batched path, exact cooling: {bx['value']:.3e} zone-updates/s")
         batched = {"value": bx["value"], "ms_per_step": bx["ms_per_step"],
                    "leaf_blocks_end_total": bx["nleaf"], "persistent_grids": bx["grids"],
                    "threads_per_workgroup": bx["threads"],
@@ -405,6 +413,7 @@ def main():
         sur = None
         if model is not None and not args.no_surrogate:
             bs = run_batched(cfgs, args.steps, args.warmup, model, nstates=args.batch, **bopts)
+            _note(f"batched path, surrogate cooling: {bs['value']:.3e} zone-updates/s")
             # L1 density error of every instance against its exact run
             l1 = [l1_density(cfgs[i], bs["states"][i], bx["states"][i]) for i in range(len(cfgs))]
             sur = {"value": bs["value"], "ms_per_step": bs["ms_per_step"],
@@ -436,10 +445,13 @@ def main():
         out["step_frac"] = plain["step_frac"]
     if args.path == "both" and args.c4_steps > 0:
         out["c4"] = run_c4(args.c4_steps, 20, args.groups)
+        _note("C4 ensemble done")
     if not args.no_cpu_baseline:
         # the reference's own code where its build is present, else the oracle
         ref = cpu_baseline("reference", args.workload)
+        _note("CPU baseline (reference) done")
         port = cpu_baseline("port", args.workload)
+        _note("CPU baseline (C oracle) done")
         out["cpu_baseline"] = ref if ref is not None else port
         if ref is not None:
             out["cpu_baseline_port"] = port
