@@ -70,8 +70,8 @@ static void check_shape(QueueRef q, int grid, int bs, int persist) {
   }
 }
 int launch_stageA(const wx_params *P, int, QueueRef q, int grid, int bs, int persist, const double *U, double *Uh,
-                  DevScalars *S, hipStream_t st) {
-  hipsim::launch(st, {P, q.hdr, q.rec, U, Uh, S}, [=] { check_shape(q, grid, bs, persist); });
+                  const double *Pc, DevScalars *S, hipStream_t st) {
+  hipsim::launch(st, {P, q.hdr, q.rec, U, Uh, Pc, S}, [=] { check_shape(q, grid, bs, persist); });
   return 0;
 }
 int launch_stageB(const wx_params *P, int, QueueRef q, int grid, int bs, int persist, const int2 *ops,
@@ -81,8 +81,8 @@ int launch_stageB(const wx_params *P, int, QueueRef q, int grid, int bs, int per
 }
 int launch_fillI_cool(const wx_params *P, int, QueueRef q, int grid, int bs, int persist, const int2 *ops,
                       const BlockRec *binfo, const double *Uin, double *Uout, double *stat, int8_t *flag,
-                      DevScalars *S, FlagOut fo, hipStream_t st) {
-  hipsim::launch(st, {P, q.hdr, q.rec, ops, binfo, Uin, Uout, stat, flag, S, fo.inst_off, fo.flagq, fo.Sout}, [=] {
+                      DevScalars *S, FlagOut fo, double *Pc, hipStream_t st) {
+  hipsim::launch(st, {P, q.hdr, q.rec, ops, binfo, Uin, Uout, stat, flag, S, fo.inst_off, fo.flagq, fo.Sout, Pc}, [=] {
     check_shape(q, grid, bs, persist);
     write_flags(P, q.rec, q.hdr->nleaf, (signed char *)flag, stat);
     if (fo.flagq) {
@@ -93,8 +93,8 @@ int launch_fillI_cool(const wx_params *P, int, QueueRef q, int grid, int bs, int
   return 0;
 }
 int launch_blockstat(const wx_params *P, int, const LeafRec *leaves, int nleaf, const double *U, double *stat,
-                     int8_t *flag, hipStream_t st) {
-  hipsim::launch(st, {P, leaves, U, stat, flag}, [=] { write_flags(P, leaves, nleaf, (signed char *)flag, stat); });
+                     int8_t *flag, double *Pc, int, hipStream_t st) {
+  hipsim::launch(st, {P, leaves, U, stat, flag, Pc}, [=] { write_flags(P, leaves, nleaf, (signed char *)flag, stat); });
   return 0;
 }
 int launch_dtreduce(const wx_params *P, const LeafRec *rec, const double *stat, const int32_t *inst_off, int ninst,
@@ -116,8 +116,8 @@ int launch_gather_flags(QueueRef q, const int32_t *inst_off, int, const int8_t *
   return 0;
 }
 int launch_meshops(const wx_params *P, int, const MeshOp *ops, int nops, double *U, double *stat, int8_t *flag,
-                   hipStream_t st) {
-  hipsim::launch(st, {P, ops, U, stat, flag}, [=] {
+                   double *Pc, hipStream_t st) {
+  hipsim::launch(st, {P, ops, U, stat, flag, Pc}, [=] {
     std::lock_guard<std::mutex> g(g_mu);
     for (int k = 0; k < nops; ++k) {
       g_ops.push_back(ops[k]);
@@ -145,7 +145,7 @@ int launch_h2d(const CopyDesc *d, int n, hipStream_t st) {
 int launch_fillI_cool_sur(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const int2 *ops,
                           const BlockRec *binfo, const double *Uin, double *Uout, double *stat, int8_t *flag,
                           DevScalars *S, SurDev *, FlagOut fo, hipStream_t st) {
-  return launch_fillI_cool(P, neq, q, grid, bs, persist, ops, binfo, Uin, Uout, stat, flag, S, fo, st);
+  return launch_fillI_cool(P, neq, q, grid, bs, persist, ops, binfo, Uin, Uout, stat, flag, S, fo, nullptr, st);
 }
 int sur_upload(SurDev *, const wx_sur_model *, const wx_params *, hipStream_t) { return 0; }
 int sur_counters(SurDev *, hipStream_t, int, int64_t *c, int64_t *f) {

@@ -59,26 +59,32 @@ def _pair_run(cfg, nsteps, cap, admesh, tol=TOL, label=""):
 
 
 def test_c1_explosion_steps():
-    """C1 (adiabatic, single level, outflow, OUTFYB compat) for 40 steps."""
+    """C1 (adiabatic, single level, outflow, OUTFYB compat) for 40 steps:
+    bit-identical."""
     cfg = P.config_c1()
     w = _pair_run(cfg, 40, 64, admesh=False, label="c1")
     print("c1 worst rel err", w)
+    assert w == 0.0
 
 
 def test_c2_sn_dmc_amr_steps():
     """C2 physics (DMC cooling, 5 levels, outflow) on the full 8x8 root layout,
-    admesh every step on both sides; trees must stay identical."""
+    admesh every step on both sides; trees must stay identical, and with the
+    cooled pressure carried as the reference carries primit every cell is
+    bit-identical."""
     cfg = P.config_c2(nblocks=6000)
     w = _pair_run(cfg, 25, 6000, admesh=True, label="c2")
     print("c2 worst rel err", w)
+    assert w == 0.0
 
 
 def test_c3_jet_coolingh_amr_steps():
     """C3 physics (coolingh, jet source, reflective left/bottom) on a reduced
-    16x4 root layout, 4 levels, admesh every step."""
+    16x4 root layout, 4 levels, admesh every step: bit-identical."""
     cfg = P.config_c3(nblocks=6000, nbrootx=16, nbrooty=4)
     w = _pair_run(cfg, 25, 6000, admesh=True, label="c3")
     print("c3 worst rel err", w)
+    assert w == 0.0
 
 
 def test_c3_jet_full_size_steps():

@@ -145,16 +145,22 @@ int launch_build_queue(const QueueHdr *hdr, const int32_t *inst_off, const int32
 // bs = threads per workgroup: 256, 384 or 512 (4, 6 or 8 waves).  persist
 // = 0: workgroup g takes queue entry g (grid >= queue length); 1: workgroup
 // g takes entries g, g + grid, ... (any grid).
+// Pc (nullptr unless some instance cools with DMC): per block ID a padded
+// tile of the pressure the last DMC cooling left in cells 0..17 (the
+// reference's primit(nb,4,:,:), src/coolingdmc.f90:27-38); stage A and the
+// statistics read it, the fill writes it, k_meshops starts it for new leaves.
 int launch_stageA(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const double *U,
-                  double *Q, DevScalars *S, hipStream_t stream);
+                  double *Q, const double *Pc, DevScalars *S, hipStream_t stream);
 int launch_stageB(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const int2 *ops,
                   const BlockRec *binfo, double *U, double *Q, DevScalars *S, hipStream_t stream);
 int launch_fillI_cool(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const int2 *ops,
                       const BlockRec *binfo, const double *Uin, double *Uout, double *stat, int8_t *flag,
-                      DevScalars *S, FlagOut fo, hipStream_t stream);
+                      DevScalars *S, FlagOut fo, double *Pc, hipStream_t stream);
 // Statistics of an explicit list of leaf records (one workgroup each).
+// Under DMC they read the carried pressure, or start it from u: pc_init 1
+// for every record, 2 for the records whose pad has bit 0 set, 0 for none.
 int launch_blockstat(const wx_params *P, int neq, const LeafRec *leaves, int nleaf, const double *U,
-                     double *stat, int8_t *flag, hipStream_t stream);
+                     double *stat, int8_t *flag, double *Pc, int pc_init, hipStream_t stream);
 // Workgroup i reduces the CFL partials of instance i's queue entries
 // [inst_off[i], inst_off[i+1]).
 int launch_dtreduce(const wx_params *P, const LeafRec *rec, const double *stat, const int32_t *inst_off,
@@ -166,7 +172,7 @@ int launch_gather_flags(QueueRef q, const int32_t *inst_off, int grid, const int
 // The data parts of refine/coarse; with stat != nullptr also the block
 // statistics of the new leaves (as k_blockstat would compute them).
 int launch_meshops(const wx_params *P, int neq, const MeshOp *ops, int nops, double *U, double *stat, int8_t *flag,
-                   hipStream_t stream);
+                   double *Pc, hipStream_t stream);
 // Performs the n copies described in d (itself in mapped host memory), one
 // launch for all: a kernel reads the pinned staging directly.
 int launch_h2d(const CopyDesc *d, int n, hipStream_t stream);

@@ -53,6 +53,9 @@ struct EnsGroup : Runtime {
   DevBuf<int2> dRegOpsII, dRegOpsI, dOpsII, dOpsI;
   DevBuf<double> dStat;
   DevBuf<int8_t> dFlag;
+  // under DMC cooling: per slot, the pressure the last cooling left in cells
+  // 0..17, carried into the next stage A (nullptr without DMC instances)
+  DevBuf<double> dPc;
   // mapped pinned mirrors written by the fill's epilogue (or k_gather_flags)
   DevScalars *hS = nullptr, *hS_dev = nullptr;
   int8_t *hFlagQ = nullptr, *hFlagQ_dev = nullptr;
@@ -83,6 +86,7 @@ struct EnsGroup : Runtime {
 
   double *cur_u() { return U[cur].ptr; }
   double *other_u() { return U[cur ^ 1].ptr; }
+  double *pc() { return dPc.ptr; }
   QueueRef queue() const { return QueueRef{dQ.ptr, dRec.ptr, (int32_t)dRec.n}; }
   size_t tile() const { return (size_t)neq * NC; }
 };
@@ -201,15 +205,20 @@ int ensure_stats(EnsGroup *e) {
   for (int i = 0; i < e->n; ++i) {
     Mesh &m = e->m[i];
     const int sb = e->slotbase[i];
+    // the pad bit asks the statistics to start the carried DMC pressure
+    // from u (a state from the host)
+    const int init = m.pc_stale ? 1 : 0;
     auto add = [&](const LeafRec &r0) {
       LeafRec r = r0;
       r.id += sb;
       r.inst = i;
+      r.pad = init;
       e->dirty.push_back(r);
     };
     if (m.stats_all_stale) {
       for (const LeafRec &r : m.leaves) add(r);
       m.stats_all_stale = false;
+      m.pc_stale = false;
     } else {
       for (int id : m.new_ids)
         if (id >= 1 && id <= m.nbmax && m.lidx[id] >= 0) add(m.leaves[m.lidx[id]]);
@@ -222,7 +231,7 @@ int ensure_stats(EnsGroup *e) {
   {
     ProfScope ps(e, KID_STAT);
     rc = launch_blockstat(e->dP.ptr, e->neq, drec, (int)e->dirty.size(), e->cur_u(), e->dStat.ptr, e->dFlag.ptr,
-                          e->stream);
+                          e->pc(), 2, e->stream);
   }
   if (rc) return fail(rc, "blockstat launch failed");
   return stage_end(e);
@@ -243,7 +252,8 @@ int launch_step(EnsGroup *e, int par, bool for_graph) {
   }
   if (!rc) {
     ProfScope ps(e, prof ? KID_STAGEA : -1);
-    rc = launch_stageA(e->dP.ptr, e->neq, q, sh[0].grid, e->bs[0], sh[0].persist, Ucur, Uoth, e->dS.ptr, e->stream);
+    rc = launch_stageA(e->dP.ptr, e->neq, q, sh[0].grid, e->bs[0], sh[0].persist, Ucur, Uoth, e->pc(), e->dS.ptr,
+                       e->stream);
   }
   if (!rc) {
     ProfScope ps(e, prof ? KID_STAGEB : -1);
@@ -258,7 +268,7 @@ int launch_step(EnsGroup *e, int par, bool for_graph) {
                                  Uoth, e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, e->dSur.ptr, fo, e->stream);
     else
       rc = launch_fillI_cool(e->dP.ptr, e->neq, q, sh[2].grid, e->bs[2], sh[2].persist, e->dOpsI.ptr, e->dBinfo.ptr, Ucur,
-                             Uoth, e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, fo, e->stream);
+                             Uoth, e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, fo, e->pc(), e->stream);
   }
   if (rc) return fail(rc, "step launch failed: %s", hipGetErrorString(hipGetLastError()));
   return WX_OK;
@@ -324,7 +334,7 @@ int admesh_all(EnsGroup *e, int *changed) {
   {
     ProfScope ps(e, KID_MESH);
     rc = launch_meshops(e->dP.ptr, e->neq, dops, (int)e->ops.size(), e->cur_u(), e->dStat.ptr, e->dFlag.ptr,
-                        e->stream);
+                        e->pc(), e->stream);
   }
   if (rc) return fail(rc, "meshops launch failed");
   if ((rc = stage_end(e))) return rc;
@@ -385,6 +395,9 @@ int grp_create(const wx_params *p, int32_t n, int nthreads, EnsGroup **out) {
   if (!rc) rc = e->dOpsI.ensure(e->nrec * 8);
   if (!rc) rc = e->dStat.ensure(e->nslots * 2);
   if (!rc) rc = e->dFlag.ensure(e->nslots);
+  bool dmc = false;
+  for (int i = 0; i < n; ++i) dmc = dmc || p[i].cooling == 2;
+  if (!rc && dmc) rc = e->dPc.ensure(e->nslots * NC);
   if (rc) { grp_destroy(e); return rc; }
   if (hipHostMalloc((void **)&e->hS, (size_t)n * sizeof(DevScalars), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
       hipHostMalloc((void **)&e->hFlagQ, e->nrec, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
@@ -403,6 +416,7 @@ int grp_create(const wx_params *p, int32_t n, int nthreads, EnsGroup **out) {
   (void)hipMemsetAsync(e->dQ.ptr, 0, sizeof(QueueHdr), e->stream);
   (void)hipMemsetAsync(e->dInstOff.ptr, 0, ((size_t)n + 1) * sizeof(int32_t), e->stream);
   (void)hipMemsetAsync(e->dBinfo.ptr, 0, e->nslots * sizeof(BlockRec), e->stream);
+  if (e->dPc.ptr) (void)hipMemsetAsync(e->dPc.ptr, 0, e->dPc.n * sizeof(double), e->stream);
   (void)hipMemcpyAsync(e->dP.ptr, p, (size_t)n * sizeof(wx_params), hipMemcpyHostToDevice, e->stream);
   (void)hipMemcpyAsync(e->dLeafBase.ptr, e->leafbase.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, e->stream);
   if (hipStreamSynchronize(e->stream) != hipSuccess) { grp_destroy(e); return fail(WX_EHIP, "init sync"); }
@@ -418,7 +432,7 @@ int grp_destroy(EnsGroup *e) {
   e->dP.release(); e->dS.release(); e->dQ.release(); e->dInstOff.release(); e->dLeafBase.release();
   e->dRegRec.release(); e->dRec.release(); e->dBinfo.release(); e->dRegOpsII.release(); e->dRegOpsI.release();
   e->dOpsII.release(); e->dOpsI.release();
-  e->dStat.release(); e->dFlag.release(); e->dSur.release();
+  e->dStat.release(); e->dFlag.release(); e->dSur.release(); e->dPc.release();
   e->Runtime::destroy();
   if (e->hS) (void)hipHostFree(e->hS);
   if (e->hFlagQ) (void)hipHostFree(e->hFlagQ);
@@ -433,6 +447,7 @@ int grp_basegrid(EnsGroup *e, int32_t inst, const double *uroots, int32_t nroots
   if (!uroots || m.p.nbrootx * m.p.nbrooty != nroots)
     return fail(WX_EARG, "instance %d: nroots %d != %d x %d", e->ibase + inst, nroots, m.p.nbrootx, m.p.nbrooty);
   if ((rc = basegrid_tree(&m))) return rc;
+  m.pc_stale = true;
   const size_t bytes = (size_t)(nroots + 1) * e->tile() * sizeof(double);
   HIPCHK(hipMemcpyAsync(e->cur_u() + (size_t)e->slotbase[inst] * e->tile(), uroots, bytes, hipMemcpyHostToDevice,
                         e->stream));
@@ -567,7 +582,8 @@ int grp_tune(EnsGroup *e, int32_t reps, int32_t *grids, int32_t *threads, double
                                        e->stream) != hipSuccess) rc = fail(WX_EHIP, "tune copy");
           (void)hipEventRecord(a, e->stream);
           if (k == 0)
-            rc = launch_stageA(e->dP.ptr, e->neq, q, g, bsz, pers, e->cur_u(), e->other_u(), e->dS.ptr, e->stream);
+            rc = launch_stageA(e->dP.ptr, e->neq, q, g, bsz, pers, e->cur_u(), e->other_u(), e->pc(), e->dS.ptr,
+                               e->stream);
           else if (k == 1)
             rc = launch_stageB(e->dP.ptr, e->neq, q, g, bsz, pers, e->dOpsII.ptr, e->dBinfo.ptr, scratch.ptr, e->other_u(),
                                e->dS.ptr, e->stream);
@@ -576,8 +592,9 @@ int grp_tune(EnsGroup *e, int32_t reps, int32_t *grids, int32_t *threads, double
                                        scratch.ptr, e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, e->dSur.ptr, FlagOut{},
                                        e->stream);
           else
+            // (a trial: the carried pressure is not written)
             rc = launch_fillI_cool(e->dP.ptr, e->neq, q, g, bsz, pers, e->dOpsI.ptr, e->dBinfo.ptr, e->cur_u(), scratch.ptr,
-                                   e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, FlagOut{}, e->stream);
+                                   e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, FlagOut{}, nullptr, e->stream);
           (void)hipEventRecord(b, e->stream);
           (void)hipEventSynchronize(b);
           float ms = 0.f;

@@ -51,6 +51,11 @@ struct wx_ctx : Mesh, Runtime {
   // cooling surrogate (coolingh only), used when use_sur
   DevBuf<SurDev> dSur;
   bool use_sur = false;
+  // under DMC cooling: the pressure the last cooling left in each leaf's
+  // cells 0..17 (the reference's primit), carried into the next stage A
+  // (Mesh::pc_stale: the state came from the host)
+  DevBuf<double> dPc;
+  double *pc() { return dPc.ptr; }
   // device table builder: the tree rows on the device and the builder's
   // per-ID scan results; used after a mesh change unless the list is
   // external (wx_set_bounds) or the host builder is selected
@@ -307,7 +312,8 @@ int update_stats(wx_ctx *c) {
   if (c->stats_all_stale) {
     ProfScope ps(c, KID_STAT);
     rc = launch_blockstat(c->dP.ptr, c->neq, c->tLeaves, c->nleaf(), c->cur_u(), c->dStat.ptr,
-                          c->dFlag.ptr, c->stream);
+                          c->dFlag.ptr, c->pc(), c->pc_stale ? 1 : 0, c->stream);
+    c->pc_stale = false;
     if (rc) return fail(rc, "blockstat launch failed");
     c->stats_all_stale = false;
     c->new_ids.clear();
@@ -324,7 +330,7 @@ int update_stats(wx_ctx *c) {
   {
     ProfScope ps(c, KID_STAT);
     rc = launch_blockstat(c->dP.ptr, c->neq, drec, (int)c->dirty.size(), c->cur_u(), c->dStat.ptr, c->dFlag.ptr,
-                          c->stream);
+                          c->pc(), c->pc_stale ? 1 : 0, c->stream);
   }
   if (rc) return fail(rc, "blockstat launch failed");
   return stage_end(c);
@@ -360,8 +366,8 @@ int sync_scalars(wx_ctx *c) {
 // tstep: stage A, then stage B and the fill (which need the ops tables).
 int run_stage_a(wx_ctx *c) {
   ProfScope ps(c, KID_STAGEA);
-  int rc = launch_stageA(c->dP.ptr, c->neq, c->queue(), c->nleaf(), 256, 0, c->cur_u(), c->other_u(), c->dS.ptr,
-                         c->stream);
+  int rc = launch_stageA(c->dP.ptr, c->neq, c->queue(), c->nleaf(), 256, 0, c->cur_u(), c->other_u(), c->pc(),
+                         c->dS.ptr, c->stream);
   if (rc) return fail(rc, "stage launch failed: %s", hipGetErrorString(hipGetLastError()));
   return WX_OK;
 }
@@ -383,7 +389,7 @@ int run_stages_bf(wx_ctx *c) {
                                  c->other_u(), c->dStat.ptr, c->dFlag.ptr, c->dS.ptr, c->dSur.ptr, fo, c->stream);
     else
       rc = launch_fillI_cool(c->dP.ptr, c->neq, c->queue(), g, 256, 0, opsI, c->tBinfo, c->cur_u(),
-                             c->other_u(), c->dStat.ptr, c->dFlag.ptr, c->dS.ptr, fo, c->stream);
+                             c->other_u(), c->dStat.ptr, c->dFlag.ptr, c->dS.ptr, fo, c->pc(), c->stream);
   }
   if (rc) return fail(rc, "stage launch failed: %s", hipGetErrorString(hipGetLastError()));
   c->cur ^= 1;
@@ -415,7 +421,7 @@ int run_meshops(wx_ctx *c, std::vector<MeshOp> &ops, bool stats) {
   {
     ProfScope ps(c, KID_MESH);
     rc = launch_meshops(c->dP.ptr, c->neq, dops, (int)ops.size(), c->cur_u(), stats ? c->dStat.ptr : nullptr,
-                        stats ? c->dFlag.ptr : nullptr, c->stream);
+                        stats ? c->dFlag.ptr : nullptr, c->pc(), c->stream);
   }
   if (rc) return fail(rc, "meshops launch failed");
   if ((rc = stage_end(c))) return rc;
@@ -496,7 +502,9 @@ int wx_create(const wx_params *p, wx_ctx **out) {
   if (!rc) rc = c->dLbInfo.ensure((size_t)c->cap + 1);
   if (!rc) rc = c->dLbPart.ensure((size_t)(c->cap + 256) / 256);
   if (!rc) rc = c->dLbLen.ensure(1);
+  if (!rc && p->cooling == 2) rc = c->dPc.ensure((size_t)(c->cap + 1) * NC);
   if (rc) { wx_destroy(c); return rc; }
+  if (c->dPc.ptr) (void)hipMemsetAsync(c->dPc.ptr, 0, c->dPc.n * sizeof(double), c->stream);
   if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) {
     c->side = nullptr;
     wx_destroy(c);
@@ -535,7 +543,7 @@ int wx_destroy(wx_ctx *c) {
   for (auto &b : c->U) b.release();
   c->dP.release(); c->dS.release(); c->dTab.release(); c->dStat.release(); c->dFlag.release();
   c->dScratch.release(); c->dSur.release(); c->dTree.release(); c->dLbInfo.release(); c->dLbPart.release();
-  c->dLbLen.release();
+  c->dLbLen.release(); c->dPc.release();
   if (c->evTree) (void)hipEventDestroy(c->evTree);
   if (c->evOps) (void)hipEventDestroy(c->evOps);
   if (c->side) (void)hipStreamDestroy(c->side);
@@ -555,6 +563,7 @@ int wx_upload(wx_ctx *c, const double *u, int32_t layout, const int32_t *lp, con
   int rc = set_tree(c, lp, leaf, icoord, ld, nbmax, nbleafs, nullptr);
   if (rc) return rc;
   c->stats_all_stale = true;
+  c->pc_stale = true;
   if (layout == WX_LAYOUT_BLOCK) {
     const size_t n = (size_t)(nbmax + 1) * c->neq * NC;
     HIPCHK(hipMemcpyAsync(c->cur_u(), u, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -750,6 +759,7 @@ int wx_basegrid(wx_ctx *c, const double *uroots, int32_t nroots) {
     return fail(WX_EARG, "nroots %d != %d x %d", nroots, c->p.nbrootx, c->p.nbrooty);
   int rc = basegrid_tree(c);
   if (rc) return rc;
+  c->pc_stale = true;
   const size_t n = (size_t)(nroots + 1) * c->neq * NC;
   HIPCHK(hipMemcpyAsync(c->cur_u(), uroots, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));

@@ -220,7 +220,7 @@ __device__ __forceinline__ void cyl_source(const wx_params *__restrict__ p, cons
 template <int NEQ, int BS, bool PERSIST>
 __global__ __launch_bounds__(BS) void k_stageA(const wx_params *__restrict__ P, QueueRef q,
                                                 const double *__restrict__ U, double *__restrict__ Q,
-                                                DevScalars *__restrict__ Sa) {
+                                                const double *__restrict__ Pc, DevScalars *__restrict__ Sa) {
   __shared__ double prim[NEQ * NC];
   __shared__ double F[NEQ * 272];
   __shared__ double G[NEQ * 272];
@@ -236,12 +236,21 @@ __global__ __launch_bounds__(BS) void k_stageA(const wx_params *__restrict__ P, 
   const wx_params *__restrict__ p = P + L.inst;
   DevScalars *__restrict__ S = Sa + L.inst;
   const double *ub = U + (size_t)L.id * NEQ * NC;
-  // rows j = 0..17 are one contiguous range [NP, 19*NP) of every variable
+  // rows j = 0..17 are one contiguous range [NP, 19*NP) of every variable.
+  // Under DMC the pressure of cells 0..17 is the one the last cooling left
+  // (the reference's primit, src/coolingdmc.f90:27-38 and
+  // src/tstep.f90:27-36), not the one u gives back.
+  const bool carried = Pc && p->cooling == 2;
+  const double *pcb = carried ? Pc + (size_t)L.id * NC : nullptr;
   for (int c = threadIdx.x; c < 18 * NP; c += blockDim.x) {
     double uu[NEQ], q[NEQ], T;
 #pragma unroll
     for (int k = 0; k < NEQ; ++k) uu[k] = ub[k * NC + NP + c];
     uprim<NEQ>(p, q, uu, T);
+    if (carried) {
+      const int col = c % NP;
+      if (col >= 1 && col <= 18) q[3] = pcb[NP + c];
+    }
 #pragma unroll
     for (int k = 0; k < NEQ; ++k) prim[k * NC + NP + c] = q[k];
   }
@@ -584,7 +593,8 @@ __global__ __launch_bounds__(BS, (SUR ? 2 : 1024 / BS)) void k_fillI_cool(const 
                                                     int8_t *__restrict__ flag,
                                                     DevScalars *__restrict__ Sa,
                                                     FlagOut fo,
-                                                    SurDev *__restrict__ M) {
+                                                    SurDev *__restrict__ M,
+                                                    double *__restrict__ Pc) {
   __shared__ double tile[NEQ * NC];
   // the network in LDS, staged once per workgroup (no LDS without the surrogate)
   __shared__ __align__(16) typename SurSlot<SUR>::type Wbuf;
@@ -668,12 +678,17 @@ __global__ __launch_bounds__(BS, (SUR ? 2 : 1024 / BS)) void k_fillI_cool(const 
     double uu[NEQ];
 #pragma unroll
     for (int k = 0; k < NEQ; ++k) uu[k] = tile[k * NC + IDX(i, j)];
+    double pcool = 0.0;
     if (!done && mode == 1) atomic_h<NEQ>(p, dts, uu);
-    else if (!done && mode == 2) cool_dmc<NEQ>(p, dts, uu);
+    else if (!done && mode == 2) cool_dmc<NEQ>(p, dts, uu, pcool);
 #pragma unroll
     for (int k = 0; k < NEQ; ++k) ob[k * NC + IDX(i, j)] = uu[k];
     double q[NEQ], T;
     uprim<NEQ>(p, q, uu, T);
+    if (mode == 2) {   // DMC: the cooled pressure, carried into the next step (src/coolingdmc.f90:27-38)
+      q[3] = pcool;
+      if (Pc) Pc[(size_t)L.id * NC + IDX(i, j)] = pcool;
+    }
 #pragma unroll
     for (int k = 0; k < NEQ; ++k) tile[k * NC + IDX(i, j)] = q[k];
   }
@@ -699,22 +714,34 @@ __global__ __launch_bounds__(BS, (SUR ? 2 : 1024 / BS)) void k_fillI_cool(const 
 
 // Statistics only (after upload or a mesh change).
 template <int NEQ>
+// Under DMC (Pc != nullptr, the instance cooling with DMC) the pressure is
+// the carried one, or (pc_init 1, or 2 and the record's pad bit 0) the
+// block's state came from outside and the carried pressure starts as the
+// one u gives (the reference's calcprim after basegrid or a warm start).
 __global__ __launch_bounds__(256) void k_blockstat(const wx_params *__restrict__ P,
                                                    const LeafRec *__restrict__ leaves,
                                                    const double *__restrict__ U,
                                                    double *__restrict__ stat,
-                                                   int8_t *__restrict__ flag) {
+                                                   int8_t *__restrict__ flag,
+                                                   double *__restrict__ Pc, int pc_init) {
   __shared__ double tile[NEQ * NC];
   const int l = blockIdx.x;
   const LeafRec L = leaves[l];
   const wx_params *__restrict__ p = P + L.inst;
   const double *ub = U + (size_t)L.id * NEQ * NC;
+  const bool carried = Pc && p->cooling == 2;
+  const bool init = pc_init == 1 || (pc_init == 2 && (L.pad & 1));
+  double *pcb = carried ? Pc + (size_t)L.id * NC : nullptr;
   for (int c = threadIdx.x; c < 18 * 18; c += blockDim.x) {
     const int i = c % 18, j = c / 18;
     double uu[NEQ], q[NEQ], T;
 #pragma unroll
     for (int k = 0; k < NEQ; ++k) uu[k] = ub[k * NC + IDX(i, j)];
     uprim<NEQ>(p, q, uu, T);
+    if (carried) {
+      if (init) pcb[IDX(i, j)] = q[3];
+      else q[3] = pcb[IDX(i, j)];
+    }
 #pragma unroll
     for (int k = 0; k < NEQ; ++k) tile[k * NC + IDX(i, j)] = q[k];
   }
@@ -814,7 +841,7 @@ __global__ __launch_bounds__(256) void k_gather_flags(QueueRef q, const int32_t 
 template <int NEQ>
 __global__ __launch_bounds__(256) void k_meshops(const wx_params *__restrict__ P, const MeshOp *__restrict__ ops,
                                                  double *__restrict__ U, double *__restrict__ stat,
-                                                 int8_t *__restrict__ flag) {
+                                                 int8_t *__restrict__ flag, double *__restrict__ Pc) {
   // workgroup 4 * o + s: son s of a refine op o; a coarse op's dad is done
   // by its first workgroup
   const MeshOp op = ops[blockIdx.x >> 2];
@@ -833,10 +860,13 @@ __global__ __launch_bounds__(256) void k_meshops(const wx_params *__restrict__ P
 #pragma unroll
       for (int k = 0; k < NEQ; ++k) sb[k * NC + cc] = pb[k * NC + IDX(si, sj)];
     }
-    if (!stat) return;
     // statistics of the son from its cells 0..17, which are the parent's
-    // cells it was injected from (read back from the parent, not the son)
+    // cells it was injected from (read back from the parent, not the son);
+    // under DMC its carried pressure starts from them (src/refine.f90:231-258
+    // recomputes the sons' primit)
     const wx_params *__restrict__ p = P + op.inst;
+    double *pcb = (Pc && p->cooling == 2) ? Pc + (size_t)son * NC : nullptr;
+    if (!stat && !pcb) return;
     for (int c = tid; c < 18 * 18; c += blockDim.x) {
       const int i = c % 18, j = c / 18;
       const int si = (i + 1) / 2 + ((s & 1) ? NX / 2 : 0);
@@ -845,9 +875,11 @@ __global__ __launch_bounds__(256) void k_meshops(const wx_params *__restrict__ P
 #pragma unroll
       for (int k = 0; k < NEQ; ++k) uu[k] = pb[k * NC + IDX(si, sj)];
       uprim<NEQ>(p, q, uu, T);
+      if (pcb) pcb[IDX(i, j)] = q[3];
 #pragma unroll
       for (int k = 0; k < NEQ; ++k) d[k * NC + IDX(i, j)] = q[k];
     }
+    if (!stat) return;
     __syncthreads();
     LeafRec L{};
     L.id = son;
@@ -890,9 +922,12 @@ __global__ __launch_bounds__(256) void k_meshops(const wx_params *__restrict__ P
     __syncthreads();
     double *db = U + (size_t)op.a * NEQ * NC;
     for (int c = tid; c < NEQ * NC; c += blockDim.x) db[c] = d[c];
-    if (!stat) return;
-    // statistics of the dad: primitives of its cells 0..17 in place
+    // statistics of the dad: primitives of its cells 0..17 in place; under
+    // DMC its carried pressure starts from them (src/coarse.f90:178, 360
+    // recompute the dad's primit)
     const wx_params *__restrict__ p = P + op.inst;
+    double *pcb = (Pc && p->cooling == 2) ? Pc + (size_t)op.a * NC : nullptr;
+    if (!stat && !pcb) return;
     __syncthreads();
     for (int c = tid; c < 18 * 18; c += blockDim.x) {
       const int i = c % 18, j = c / 18;
@@ -900,9 +935,11 @@ __global__ __launch_bounds__(256) void k_meshops(const wx_params *__restrict__ P
 #pragma unroll
       for (int k = 0; k < NEQ; ++k) uu[k] = d[k * NC + IDX(i, j)];
       uprim<NEQ>(p, q, uu, T);
+      if (pcb) pcb[IDX(i, j)] = q[3];
 #pragma unroll
       for (int k = 0; k < NEQ; ++k) d[k * NC + IDX(i, j)] = q[k];
     }
+    if (!stat) return;
     __syncthreads();
     LeafRec L{};
     L.id = op.a;
@@ -987,9 +1024,9 @@ int launch_build_queue(const QueueHdr *hdr, const int32_t *inst_off, const int32
   else { constexpr bool PS = false; CALL; }
 
 int launch_stageA(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const double *U,
-                  double *Q, DevScalars *S, hipStream_t stream) {
+                  double *Q, const double *Pc, DevScalars *S, hipStream_t stream) {
   if (grid <= 0) return WX_OK;
-  WX_PERSIST_DISPATCH(persist, WX_BS_DISPATCH(bs, WX_DISPATCH(neq, hipLaunchKernelGGL((k_stageA<N, B, PS>), dim3(grid), dim3(B), 0, stream, P, q, U, Q, S))));
+  WX_PERSIST_DISPATCH(persist, WX_BS_DISPATCH(bs, WX_DISPATCH(neq, hipLaunchKernelGGL((k_stageA<N, B, PS>), dim3(grid), dim3(B), 0, stream, P, q, U, Q, Pc, S))));
   return hip_status();
 }
 
@@ -1002,9 +1039,9 @@ int launch_stageB(const wx_params *P, int neq, QueueRef q, int grid, int bs, int
 
 int launch_fillI_cool(const wx_params *P, int neq, QueueRef q, int grid, int bs, int persist, const int2 *ops,
                       const BlockRec *binfo, const double *Uin, double *Uout, double *stat, int8_t *flag,
-                      DevScalars *S, FlagOut fo, hipStream_t stream) {
+                      DevScalars *S, FlagOut fo, double *Pc, hipStream_t stream) {
   if (grid <= 0) return WX_OK;
-  WX_PERSIST_DISPATCH(persist, WX_BS_DISPATCH(bs, WX_DISPATCH(neq, hipLaunchKernelGGL((k_fillI_cool<N, false, B, PS>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, fo, nullptr))));
+  WX_PERSIST_DISPATCH(persist, WX_BS_DISPATCH(bs, WX_DISPATCH(neq, hipLaunchKernelGGL((k_fillI_cool<N, false, B, PS>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, fo, nullptr, Pc))));
   return hip_status();
 }
 
@@ -1013,17 +1050,18 @@ int launch_fillI_cool_sur(const wx_params *P, int neq, QueueRef q, int grid, int
                           DevScalars *S, SurDev *M, FlagOut fo, hipStream_t stream) {
   if (grid <= 0) return WX_OK;
   switch (neq) {   // coolingh needs u(5) = n_HI
-    case 5: WX_PERSIST_DISPATCH(persist, WX_BS_DISPATCH(bs, hipLaunchKernelGGL((k_fillI_cool<5, true, B, PS>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, fo, M))); break;
-    case 6: WX_PERSIST_DISPATCH(persist, WX_BS_DISPATCH(bs, hipLaunchKernelGGL((k_fillI_cool<6, true, B, PS>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, fo, M))); break;
+    case 5: WX_PERSIST_DISPATCH(persist, WX_BS_DISPATCH(bs, hipLaunchKernelGGL((k_fillI_cool<5, true, B, PS>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, fo, M, nullptr))); break;
+    case 6: WX_PERSIST_DISPATCH(persist, WX_BS_DISPATCH(bs, hipLaunchKernelGGL((k_fillI_cool<6, true, B, PS>), dim3(grid), dim3(B), 0, stream, P, q, ops, binfo, Uin, Uout, stat, flag, S, fo, M, nullptr))); break;
     default: return WX_EARG;
   }
   return hip_status();
 }
 
 int launch_blockstat(const wx_params *P, int neq, const LeafRec *leaves, int nleaf, const double *U,
-                     double *stat, int8_t *flag, hipStream_t stream) {
+                     double *stat, int8_t *flag, double *Pc, int pc_init, hipStream_t stream) {
   if (nleaf <= 0) return WX_OK;
-  WX_DISPATCH(neq, hipLaunchKernelGGL(k_blockstat<N>, dim3(nleaf), dim3(256), 0, stream, P, leaves, U, stat, flag));
+  WX_DISPATCH(neq, hipLaunchKernelGGL(k_blockstat<N>, dim3(nleaf), dim3(256), 0, stream, P, leaves, U, stat, flag, Pc,
+                                      pc_init));
   return hip_status();
 }
 
@@ -1043,9 +1081,9 @@ int launch_gather_flags(QueueRef q, const int32_t *inst_off, int grid, const int
 }
 
 int launch_meshops(const wx_params *P, int neq, const MeshOp *ops, int nops, double *U, double *stat, int8_t *flag,
-                   hipStream_t stream) {
+                   double *Pc, hipStream_t stream) {
   if (nops <= 0) return WX_OK;
-  WX_DISPATCH(neq, hipLaunchKernelGGL(k_meshops<N>, dim3(4 * nops), dim3(256), 0, stream, P, ops, U, stat, flag));
+  WX_DISPATCH(neq, hipLaunchKernelGGL(k_meshops<N>, dim3(4 * nops), dim3(256), 0, stream, P, ops, U, stat, flag, Pc));
   return hip_status();
 }
 

@@ -333,9 +333,10 @@ __device__ __forceinline__ double cooldmc(const wx_params *__restrict__ p, doubl
 
 // src/coolingdmc.f90:19-50 on one cell's conserved state
 template <int NEQ>
-__device__ __forceinline__ void cool_dmc(const wx_params *__restrict__ p, double dt, double *uu) {
+__device__ __forceinline__ void cool_dmc(const wx_params *__restrict__ p, double dt, double *uu, double &pcool) {
   double q[NEQ], T;
   uprim<NEQ>(p, q, uu, T);
+  pcool = q[3];
   if (T > 1.e4) {
     double Eth0 = p->cv * q[3];
     double Aloss = cooldmc(p, T);
@@ -343,6 +344,7 @@ __device__ __forceinline__ void cool_dmc(const wx_params *__restrict__ p, double
     double Ce = (Aloss * (dens * dens)) / (Eth0 * p->Psc);
     double pn = q[3] * libm::wexp(-(Ce * dt));
     uu[3] = uu[3] - Eth0 + p->cv * pn;
+    pcool = pn;   // the primitive the reference keeps (src/coolingdmc.f90:38)
   }
 }
 

@@ -81,6 +81,10 @@ struct Mesh {
   // block statistics (CFL partials + refinement flag) are kept per block ID;
   // after a mesh change only the new leaves need them recomputed
   bool stats_all_stale = true;
+  // under DMC cooling: the state came from the host (basegrid, upload), so
+  // the pressure carried from cooling to the next stage A must first be
+  // taken from u, as the reference's calcprim does
+  bool pc_stale = true;
   std::vector<int> new_ids;
   // admesh scratch flags per block ID (all zero between calls), and its lists
   std::vector<uint8_t> up, dn;
