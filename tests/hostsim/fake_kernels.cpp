@@ -56,7 +56,10 @@ int launch_build_queue(const QueueHdr *hdr, const int32_t *inst_off, const int32
       while (i + 1 < hdr->ninst && inst_off[i + 1] <= l) ++i;
       const size_t src = (size_t)leafbase[i] + (size_t)(l - inst_off[i]);
       rec[l] = reg_rec[src];
-      for (int k = 0; k < 8; ++k) { ops2[(size_t)l * 8 + k] = reg_ops2[src * 8 + k]; ops1[(size_t)l * 8 + k] = reg_ops1[src * 8 + k]; }
+      for (int k = 0; k < 8; ++k) {
+        ops2[(size_t)l * 8 + k] = reg_ops2[src * 8 + k];
+        if (ops1) ops1[(size_t)l * 8 + k] = reg_ops1[src * 8 + k];
+      }
     }
   });
   return 0;

@@ -175,6 +175,9 @@ static int run_ensemble(int rounds, unsigned seed, int groups) {
     p.neq = neq; p.nlevs = nl[i]; p.nbrootx = nrx[i]; p.nbrooty = nry[i]; p.nblocks = caps[i];
     p.cv = 1.5; p.gamma = 5. / 3.; p.cfl = 0.4; p.eta = 1e-3; p.tol = 0.05; p.tol2 = 1.0;
     p.compat = 3; p.Tempsc = 1; p.Psc = 1; p.tsc = 1;
+    // seeds 1 mod 4: reflective bottoms, so no instance is in the OUTFYB mode
+    // and fill I's ops are fill II's for the whole ensemble
+    if (seed % 4 == 1) p.bc[2] = 1;
     for (int l = 0; l < nl[i]; ++l) { p.dx[l] = 1.0 / (16 * nrx[i]) / (1 << l); p.dy[l] = 1.0 / (16 * nry[i]) / (1 << l); }
     p.jet[11] = grp[i];
     base[i] = gslots[grp[i]];

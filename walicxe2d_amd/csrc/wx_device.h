@@ -138,7 +138,7 @@ struct CopyDesc {
 // workgroups of the persistent stage kernels.
 // Compacts the per-instance regions (records and both op tables, instance i
 // at region offset leafbase[i], inst_off[i+1] - inst_off[i] entries) into
-// the queue arrays in instance order.
+// the queue arrays in instance order (fill I's ops only when ops1 is set).
 int launch_build_queue(const QueueHdr *hdr, const int32_t *inst_off, const int32_t *leafbase,
                        const LeafRec *reg_rec, const int2 *reg_ops2, const int2 *reg_ops1, LeafRec *rec,
                        int2 *ops2, int2 *ops1, int grid, hipStream_t stream);

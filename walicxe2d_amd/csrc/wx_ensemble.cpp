@@ -51,6 +51,15 @@ struct EnsGroup : Runtime {
   DevBuf<LeafRec> dRegRec, dRec;          // per-instance regions, compacted queue
   DevBuf<BlockRec> dBinfo;
   DevBuf<int2> dRegOpsII, dRegOpsI, dOpsII, dOpsI;
+  // no instance in the OUTFYB compat mode: fill I's ops equal fill II's for
+  // every instance, so they are neither uploaded nor gathered, and the fill
+  // reads fill II's (fixed by the parameters for the ensemble's life)
+  bool opsI_same_all = false;
+  // block records on the device per instance: IDs 0 .. binfo_dev[i] - 1 (an
+  // ID's level and position never change once created, so after a mesh
+  // change only the IDs above the old maximum go up)
+  std::vector<size_t> binfo_dev;
+  const int2 *opsI() const { return opsI_same_all ? dOpsII.ptr : dOpsI.ptr; }
   DevBuf<double> dStat;
   DevBuf<int8_t> dFlag;
   // under DMC cooling: per slot, the pressure the last cooling left in cells
@@ -143,10 +152,14 @@ int ensure_queue(EnsGroup *e) {
   e->inst_off[0] = 0;
   for (int i = 0; i < e->n; ++i) e->inst_off[i + 1] = e->inst_off[i] + (int32_t)e->m[i].leaves.size();
   e->nleaf_total = e->inst_off[e->n];
+  const bool two = !e->opsI_same_all;
+  // block records from binfo_dev[i] on (every ID above the old maximum)
+  auto bi_first = [&](int i) { return std::min(e->binfo_dev[i], e->m[i].binfo.size()); };
   size_t total = al256(sizeof(QueueHdr)) + al256((size_t)(e->n + 1) * sizeof(int32_t));
   for (int i : stale) {
     const size_t nl = e->m[i].leaves.size();
-    total += al256(nl * sizeof(LeafRec)) + 2 * al256(nl * 8 * sizeof(int2)) + al256(e->m[i].binfo.size() * sizeof(BlockRec));
+    total += al256(nl * sizeof(LeafRec)) + (two ? 2 : 1) * al256(nl * 8 * sizeof(int2)) +
+             al256((e->m[i].binfo.size() - bi_first(i)) * sizeof(BlockRec));
   }
   if ((rc = stage_begin(e, total))) return rc;
   struct Chunk { char *rec, *o2, *o1, *bi; };
@@ -156,8 +169,8 @@ int ensure_queue(EnsGroup *e) {
     const size_t nl = m.leaves.size();
     ch[k].rec = stage_take(e, nl * sizeof(LeafRec));
     ch[k].o2 = stage_take(e, nl * 8 * sizeof(int2));
-    ch[k].o1 = stage_take(e, nl * 8 * sizeof(int2));
-    ch[k].bi = stage_take(e, m.binfo.size() * sizeof(BlockRec));
+    ch[k].o1 = two ? stage_take(e, nl * 8 * sizeof(int2)) : nullptr;
+    ch[k].bi = stage_take(e, (m.binfo.size() - bi_first(stale[k])) * sizeof(BlockRec));
   }
   e->pool->run((int)stale.size(), [&](int k) {   // local IDs -> global slots
     const int i = stale[k];
@@ -170,11 +183,11 @@ int ensure_queue(EnsGroup *e) {
       r[l].inst = i;
     }
     int2 *o2 = (int2 *)ch[k].o2, *o1 = (int2 *)ch[k].o1;
-    for (size_t t = 0; t < m.opsII.size(); ++t) {
-      o2[t] = int2{m.opsII[t].x + sb, m.opsII[t].y};
-      o1[t] = int2{m.opsI[t].x + sb, m.opsI[t].y};
-    }
-    memcpy(ch[k].bi, m.binfo.data(), m.binfo.size() * sizeof(BlockRec));
+    for (size_t t = 0; t < m.opsII.size(); ++t) o2[t] = int2{m.opsII[t].x + sb, m.opsII[t].y};
+    if (o1)
+      for (size_t t = 0; t < m.opsI.size(); ++t) o1[t] = int2{m.opsI[t].x + sb, m.opsI[t].y};
+    const size_t b0 = bi_first(i);
+    memcpy(ch[k].bi, m.binfo.data() + b0, (m.binfo.size() - b0) * sizeof(BlockRec));
   });
   for (size_t k = 0; k < stale.size(); ++k) {
     const int i = stale[k];
@@ -182,15 +195,21 @@ int ensure_queue(EnsGroup *e) {
     const size_t nl = m.leaves.size();
     if ((rc = stage_push(e, e->dRegRec.ptr + e->leafbase[i], ch[k].rec, nl * sizeof(LeafRec)))) return rc;
     if ((rc = stage_push(e, e->dRegOpsII.ptr + (size_t)e->leafbase[i] * 8, ch[k].o2, nl * 8 * sizeof(int2)))) return rc;
-    if ((rc = stage_push(e, e->dRegOpsI.ptr + (size_t)e->leafbase[i] * 8, ch[k].o1, nl * 8 * sizeof(int2)))) return rc;
-    if ((rc = stage_push(e, e->dBinfo.ptr + e->slotbase[i], ch[k].bi, m.binfo.size() * sizeof(BlockRec)))) return rc;
+    if (two && (rc = stage_push(e, e->dRegOpsI.ptr + (size_t)e->leafbase[i] * 8, ch[k].o1, nl * 8 * sizeof(int2))))
+      return rc;
+    const size_t b0 = bi_first(i);
+    if (m.binfo.size() > b0 &&
+        (rc = stage_push(e, e->dBinfo.ptr + e->slotbase[i] + b0, ch[k].bi, (m.binfo.size() - b0) * sizeof(BlockRec))))
+      return rc;
   }
   const QueueHdr qh{e->nleaf_total, e->n, {0, 0}};
   if ((rc = stage_copy(e, e->dQ.ptr, &qh, sizeof qh))) return rc;
   if ((rc = stage_copy(e, e->dInstOff.ptr, e->inst_off.data(), (size_t)(e->n + 1) * sizeof(int32_t)))) return rc;
+  for (int i : stale) e->binfo_dev[i] = 0;   // unknown until the copies are launched
   if ((rc = stage_end(e))) return rc;
+  for (int i : stale) e->binfo_dev[i] = e->m[i].binfo.size();
   rc = launch_build_queue(e->dQ.ptr, e->dInstOff.ptr, e->dLeafBase.ptr, e->dRegRec.ptr, e->dRegOpsII.ptr,
-                          e->dRegOpsI.ptr, e->dRec.ptr, e->dOpsII.ptr, e->dOpsI.ptr,
+                          two ? e->dRegOpsI.ptr : nullptr, e->dRec.ptr, e->dOpsII.ptr, two ? e->dOpsI.ptr : nullptr,
                           std::max(1, std::min((8 * e->nleaf_total + 255) / 256, 4096)), e->stream);
   if (rc) return fail(rc, "queue build launch failed");
   return WX_OK;
@@ -264,10 +283,10 @@ int launch_step(EnsGroup *e, int par, bool for_graph) {
     ProfScope ps(e, prof ? KID_FILLI : -1);
     const FlagOut fo{e->dInstOff.ptr, e->hFlagQ_dev, e->hS_dev, e->n};
     if (e->use_sur)
-      rc = launch_fillI_cool_sur(e->dP.ptr, e->neq, q, sh[2].grid, e->bs[2], sh[2].persist, e->dOpsI.ptr, e->dBinfo.ptr, Ucur,
+      rc = launch_fillI_cool_sur(e->dP.ptr, e->neq, q, sh[2].grid, e->bs[2], sh[2].persist, e->opsI(), e->dBinfo.ptr, Ucur,
                                  Uoth, e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, e->dSur.ptr, fo, e->stream);
     else
-      rc = launch_fillI_cool(e->dP.ptr, e->neq, q, sh[2].grid, e->bs[2], sh[2].persist, e->dOpsI.ptr, e->dBinfo.ptr, Ucur,
+      rc = launch_fillI_cool(e->dP.ptr, e->neq, q, sh[2].grid, e->bs[2], sh[2].persist, e->opsI(), e->dBinfo.ptr, Ucur,
                              Uoth, e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, fo, e->pc(), e->stream);
   }
   if (rc) return fail(rc, "step launch failed: %s", hipGetErrorString(hipGetLastError()));
@@ -366,6 +385,9 @@ int grp_create(const wx_params *p, int32_t n, int nthreads, EnsGroup **out) {
   e->rcs.assign(n, 0);
   e->errs.resize(n);
   e->iops.resize(n);
+  e->binfo_dev.assign(n, 0);
+  e->opsI_same_all = true;
+  for (int i = 0; i < n; ++i) e->opsI_same_all = e->opsI_same_all && !lb_outfyb(p[i]);
   for (int i = 0; i < n; ++i) {
     e->m[i].init(p[i]);
     e->slotbase[i] = (int32_t)e->nslots;
@@ -390,9 +412,9 @@ int grp_create(const wx_params *p, int32_t n, int nthreads, EnsGroup **out) {
   if (!rc) rc = e->dRec.ensure(e->nrec);
   if (!rc) rc = e->dBinfo.ensure(e->nslots);
   if (!rc) rc = e->dRegOpsII.ensure(e->nrec * 8);
-  if (!rc) rc = e->dRegOpsI.ensure(e->nrec * 8);
+  if (!rc && !e->opsI_same_all) rc = e->dRegOpsI.ensure(e->nrec * 8);
   if (!rc) rc = e->dOpsII.ensure(e->nrec * 8);
-  if (!rc) rc = e->dOpsI.ensure(e->nrec * 8);
+  if (!rc && !e->opsI_same_all) rc = e->dOpsI.ensure(e->nrec * 8);
   if (!rc) rc = e->dStat.ensure(e->nslots * 2);
   if (!rc) rc = e->dFlag.ensure(e->nslots);
   bool dmc = false;
@@ -448,6 +470,7 @@ int grp_basegrid(EnsGroup *e, int32_t inst, const double *uroots, int32_t nroots
     return fail(WX_EARG, "instance %d: nroots %d != %d x %d", e->ibase + inst, nroots, m.p.nbrootx, m.p.nbrooty);
   if ((rc = basegrid_tree(&m))) return rc;
   m.pc_stale = true;
+  e->binfo_dev[inst] = 0;
   const size_t bytes = (size_t)(nroots + 1) * e->tile() * sizeof(double);
   HIPCHK(hipMemcpyAsync(e->cur_u() + (size_t)e->slotbase[inst] * e->tile(), uroots, bytes, hipMemcpyHostToDevice,
                         e->stream));
@@ -588,12 +611,12 @@ int grp_tune(EnsGroup *e, int32_t reps, int32_t *grids, int32_t *threads, double
             rc = launch_stageB(e->dP.ptr, e->neq, q, g, bsz, pers, e->dOpsII.ptr, e->dBinfo.ptr, scratch.ptr, e->other_u(),
                                e->dS.ptr, e->stream);
           else if (e->use_sur)
-            rc = launch_fillI_cool_sur(e->dP.ptr, e->neq, q, g, bsz, pers, e->dOpsI.ptr, e->dBinfo.ptr, e->cur_u(),
+            rc = launch_fillI_cool_sur(e->dP.ptr, e->neq, q, g, bsz, pers, e->opsI(), e->dBinfo.ptr, e->cur_u(),
                                        scratch.ptr, e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, e->dSur.ptr, FlagOut{},
                                        e->stream);
           else
             // (a trial: the carried pressure is not written)
-            rc = launch_fillI_cool(e->dP.ptr, e->neq, q, g, bsz, pers, e->dOpsI.ptr, e->dBinfo.ptr, e->cur_u(), scratch.ptr,
+            rc = launch_fillI_cool(e->dP.ptr, e->neq, q, g, bsz, pers, e->opsI(), e->dBinfo.ptr, e->cur_u(), scratch.ptr,
                                    e->dStat.ptr, e->dFlag.ptr, e->dS.ptr, FlagOut{}, nullptr, e->stream);
           (void)hipEventRecord(b, e->stream);
           (void)hipEventSynchronize(b);

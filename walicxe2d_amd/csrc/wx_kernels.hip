@@ -814,7 +814,7 @@ __global__ __launch_bounds__(256) void k_build_queue(const QueueHdr *__restrict_
     const size_t src = (size_t)leafbase[lo] + (size_t)(l - inst_off[lo]);
     if (k == 0) rec[l] = reg_rec[src];
     ops2[(size_t)l * 8 + k] = reg_ops2[src * 8 + k];
-    ops1[(size_t)l * 8 + k] = reg_ops1[src * 8 + k];
+    if (ops1) ops1[(size_t)l * 8 + k] = reg_ops1[src * 8 + k];
   }
 }
 
