@@ -11,6 +11,8 @@ import subprocess
 import numpy as np
 import pytest
 
+from _hostsim_build import make as make_hostsim
+
 from walicxe2d_amd import binio
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "hostsim")
@@ -20,7 +22,7 @@ pytestmark = pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not ins
 @pytest.fixture(scope="module")
 def bindir(tmp_path_factory):
     d = str(tmp_path_factory.mktemp("bin"))
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    make_hostsim()
     r = subprocess.run([os.path.join(HERE, "hostsim_main"), "16", "3", "bin", d], capture_output=True, text=True,
                        timeout=600)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
@@ -103,7 +105,7 @@ def test_writer_matches_reference_output(variant, seed, steps, tmp_path):
     reference built from its sources in oracle/_ref where present, else the
     digests it wrote (tests/golden/make_golden_bin.py).  Capacity and the
     .true. encoding are the reference build's (4096 blocks, flang's 1)."""
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    make_hostsim()
     O = Oracle()
     cfg, pp, m = B.state(O, variant, seed, steps)
     B.write_state_file(str(tmp_path / "state.bin"), m, cfg)

@@ -10,13 +10,15 @@ import subprocess
 
 import pytest
 
+from _hostsim_build import make as make_hostsim
+
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "hostsim")
 
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not installed")
 @pytest.mark.parametrize("seed", [1, 7])
 def test_host_tree_matches_oracle_under_asan(seed):
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    make_hostsim()
     r = subprocess.run([os.path.join(HERE, "hostsim_main"), "250", str(seed)], capture_output=True, text=True,
                        timeout=600)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
@@ -33,7 +35,7 @@ def test_ensemble_trees_match_oracle_under_asan(groups):
     changes): every instance's tree after every admesh equals the oracle's for
     the same flags, no allocation a live graph references is freed, and no
     copy or memset is captured."""
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    make_hostsim()
     r = subprocess.run([os.path.join(HERE, "hostsim_main"), "120", "5", "ens", str(groups)], capture_output=True,
                        text=True, timeout=600, env={**os.environ, "WX_HOST_THREADS": "4"})
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
@@ -46,7 +48,7 @@ def test_ensemble_trees_match_oracle_under_asan(groups):
 def test_grouped_ensemble_under_tsan(groups):
     """The grouped ensemble under ThreadSanitizer: the groups' host threads
     and their worker pools share no unsynchronised state."""
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    make_hostsim()
     r = subprocess.run([os.path.join(HERE, "hostsim_tsan"), "40", "3", "ens", str(groups)], capture_output=True,
                        text=True, timeout=600, env={**os.environ, "WX_HOST_THREADS": "6"})
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
@@ -60,7 +62,7 @@ def test_ghost_tables_match_sequential_list_semantics():
     derived from it plus the OUTFYB writes) equal a direct application of the
     ordered list, entry by entry, on 320 random meshes covering the jet
     source, the OUTFYB compat mode and both boundary kinds (ASan build)."""
-    subprocess.run(["make", "-s", "-C", HERE, "tables_check"], check=True)
+    make_hostsim("tables_check")
     r = subprocess.run([os.path.join(HERE, "tables_check")], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
     assert "tables identical" in r.stdout
