@@ -123,11 +123,14 @@ __device__ void ghost_gather(const wx_params *__restrict__ p, const double *__re
   if (dimp && in_jet(p, dlev, dicx, dicy, di, dj)) jet_state<NEQ>(p, vjet, g);
 }
 
+// (Selects rather than branches: branches that assign i or j make the
+// compiler keep the pair in scratch memory.)
 __device__ __forceinline__ void ghost_coords(int side, int layer, int pos, int &i, int &j) {
-  if (side == 0) { i = 1 - layer; j = pos; }
-  else if (side == 1) { i = NX + layer; j = pos; }
-  else if (side == 2) { j = 1 - layer; i = pos; }
-  else { j = NY + layer; i = pos; }
+  static_assert(NX == NY, "one edge index for both directions");
+  const int edge = (side & 1) ? NX + layer : 1 - layer;
+  const bool xside = side < 2;
+  i = xside ? edge : pos;
+  j = xside ? pos : edge;
 }
 
 // Flux sweep shared by both stages.  prim: LDS tile [NEQ][NC].  F: x-faces
@@ -583,8 +586,14 @@ __device__ __forceinline__ void cool_cell(int c, int &i, int &j) {
   }
 }
 
+// Waves per SIMD the plain one-entry-per-workgroup fill is compiled for: 5
+// caps it at 96 VGPRs (a few spills), which measured faster than 4 waves at
+// 109 VGPRs (16.8 vs 17.8 ns per block on the 16-variant cooling jet).  The
+// persistent form carries the next record as well and keeps 1024 / BS.
+constexpr int kFillWaves = 5;
+
 template <int NEQ, bool SUR, int BS, bool PERSIST>
-__global__ __launch_bounds__(BS, (SUR ? 2 : 1024 / BS)) void k_fillI_cool(const wx_params *__restrict__ P, QueueRef q,
+__global__ __launch_bounds__(BS, (SUR ? 2 : (PERSIST ? 1024 / BS : kFillWaves))) void k_fillI_cool(const wx_params *__restrict__ P, QueueRef q,
                                                     const int2 *__restrict__ ops,
                                                     const BlockRec *__restrict__ binfo,
                                                     const double *__restrict__ Uin,
@@ -636,9 +645,11 @@ __global__ __launch_bounds__(BS, (SUR ? 2 : 1024 / BS)) void k_fillI_cool(const 
   }
   // ghost layer 1 (64 cells) from the list; a cell no entry writes keeps the
   // U^{n+1/2} ring that stage B left in this tile's ring of Uout, imposed
-  // where listed
-  if (tid < 64) {
-    const int side = tid >> 4, pos = (tid & 15) + 1;
+  // where listed.  Ghost g is gathered by the thread that cools it below:
+  // thread g with 256 threads, thread 256 + g with more.
+  const int gt = BS > 256 ? tid - 256 : tid;
+  if (gt >= 0 && gt < 64) {
+    const int side = gt >> 4, pos = (gt & 15) + 1;
     const int2 op = ops[r * 8 + side * 2 + ((pos - 1) >> 3)];
     int gi, gj;
     ghost_coords(side, 1, pos, gi, gj);
@@ -656,16 +667,15 @@ __global__ __launch_bounds__(BS, (SUR ? 2 : 1024 / BS)) void k_fillI_cool(const 
   // cooling over cells 0..n+1 (src/coolingh.f90:19-20, src/coolingdmc.f90:23-24)
   // except the four corner cells, which nothing reads: the interior (cell
   // c < 256) and the non-corner ghosts of layer 1 (c = 256 + the gather's
-  // index).  With 256 threads a ghost is cooled by the thread that gathered
-  // it, so the other waves start cooling the interior without waiting.
+  // index).  Every cell is cooled by the thread that loaded it, so no wave
+  // waits for another's loads.
   // Each cooled cell goes to Uout, and its primitives to the LDS tile for
   // the epilogue's 5-point stencils.
-  if (BS > 256) __syncthreads();
   const int mode = p->cooling;
   bool done = false;
   if constexpr (SUR && NEQ >= 5) {
     if (mode == 1) {
-      if (BS == 256) __syncthreads();
+      __syncthreads();   // the surrogate's cells are spread differently
       cool_surrogate<NEQ>(p, M, W, S->dt * p->tsc, tile);
       __syncthreads();
       done = true;
