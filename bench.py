@@ -311,7 +311,7 @@ def run_batched(cfgs, steps, warmup, model, nstates=2, groups=4, tune=True, grap
         ens.set_surrogate(model)
     if warmup:
         ens.step(warmup)
-    grids, threads = [0, 0, 0], [256, 256, 256]
+    grids, threads = [0, 0, 0], [512, 256, 256]   # the library's default shapes
     if tune:
         grids, threads, _ = ens.tune(reps=3)
     ens.set_graph(graph)

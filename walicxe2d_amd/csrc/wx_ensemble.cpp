@@ -79,9 +79,10 @@ struct EnsGroup : Runtime {
   std::vector<LeafRec> dirty;
   std::unique_ptr<WorkerPool> pool;
   // launch shapes of the stage kernels (A, B, fillI): persistent workgroups
-  // (0 = one workgroup per queue entry) and threads per workgroup
+  // (0 = one workgroup per queue entry) and threads per workgroup (stage A
+  // is fastest with 8 waves, the others with 4, on the jet ensembles)
   int grid[3] = {0, 0, 0};
-  int bs[3] = {256, 256, 256};
+  int bs[3] = {512, 256, 256};
   // queue entries the captured graphs launch workgroups for (the one-per-
   // entry kernels): recaptured when the queue outgrows it or shrinks to half
   int gspan = 0;

@@ -364,9 +364,12 @@ int sync_scalars(wx_ctx *c) {
 }
 
 // tstep: stage A, then stage B and the fill (which need the ops tables).
+// Stage A runs in 8-wave workgroups, the others in 4-wave ones: the fastest
+// shapes on the full cooling jet (1162 leaves: stage A 14.4 ns per block at
+// 512 threads, 16.0 at 256; stage B and the fill 2-5 ns slower at 384 or 512).
 int run_stage_a(wx_ctx *c) {
   ProfScope ps(c, KID_STAGEA);
-  int rc = launch_stageA(c->dP.ptr, c->neq, c->queue(), c->nleaf(), 256, 0, c->cur_u(), c->other_u(), c->pc(),
+  int rc = launch_stageA(c->dP.ptr, c->neq, c->queue(), c->nleaf(), 512, 0, c->cur_u(), c->other_u(), c->pc(),
                          c->dS.ptr, c->stream);
   if (rc) return fail(rc, "stage launch failed: %s", hipGetErrorString(hipGetLastError()));
   return WX_OK;

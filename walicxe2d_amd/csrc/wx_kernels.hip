@@ -220,8 +220,12 @@ __device__ __forceinline__ void cyl_source(const wx_params *__restrict__ p, cons
 
 // ---------------------------------------------------------------------
 // Stage A: src/tstep.f90:22-46 (hllcfluxes choice 1 + step with dt/2).
+// The one-entry-per-workgroup form is compiled for 8 waves per SIMD (at 512
+// threads that caps it at 64 VGPRs, with a few spills: 8.5 vs 9.2 ns per
+// block at 74 VGPRs and 6 waves on the 16-variant cooling jet; the smaller
+// workgroups are held to fewer waves by their LDS anyway).
 template <int NEQ, int BS, bool PERSIST>
-__global__ __launch_bounds__(BS) void k_stageA(const wx_params *__restrict__ P, QueueRef q,
+__global__ __launch_bounds__(BS, PERSIST ? 1 : 8) void k_stageA(const wx_params *__restrict__ P, QueueRef q,
                                                 const double *__restrict__ U, double *__restrict__ Q,
                                                 const double *__restrict__ Pc, DevScalars *__restrict__ Sa) {
   __shared__ double prim[NEQ * NC];
