@@ -48,18 +48,32 @@ static void write_flags(const wx_params *P, const LeafRec *leaves, int n, signed
   }
 }
 int launch_build_queue(const QueueHdr *hdr, const int32_t *inst_off, const int32_t *leafbase,
-                       const LeafRec *reg_rec, const int2 *reg_ops2, const int2 *reg_ops1, LeafRec *rec,
+                       const int32_t *slotbase, const int32_t *reg_ids, const BlockRec *binfo,
+                       const int32_t *slot_flag, const int2 *slot_ops2, const int2 *slot_ops1, LeafRec *rec,
                        int2 *ops2, int2 *ops1, int, hipStream_t st) {
-  hipsim::launch(st, {hdr, inst_off, leafbase, reg_rec, reg_ops2, reg_ops1, rec, ops2, ops1}, [=] {
+  hipsim::launch(st, {hdr, inst_off, leafbase, slotbase, reg_ids, binfo, slot_flag, slot_ops2, slot_ops1, rec, ops2,
+                      ops1},
+                 [=] {
     for (int l = 0; l < hdr->nleaf; ++l) {
       int i = 0;
       while (i + 1 < hdr->ninst && inst_off[i + 1] <= l) ++i;
-      const size_t src = (size_t)leafbase[i] + (size_t)(l - inst_off[i]);
-      rec[l] = reg_rec[src];
+      const int lp = l - inst_off[i];
+      const size_t slot = (size_t)slotbase[i] + (size_t)reg_ids[(size_t)leafbase[i] + lp];
+      const BlockRec b = binfo[slot];
+      rec[l] = LeafRec{(int32_t)slot, b.level, b.icx, b.icy, slot_flag[slot], lp, i, 0};
       for (int k = 0; k < 8; ++k) {
-        ops2[(size_t)l * 8 + k] = reg_ops2[src * 8 + k];
-        if (ops1) ops1[(size_t)l * 8 + k] = reg_ops1[src * 8 + k];
+        ops2[(size_t)l * 8 + k] = slot_ops2[slot * 8 + k];
+        if (ops1) ops1[(size_t)l * 8 + k] = slot_ops1[slot * 8 + k];
       }
+    }
+  });
+  return 0;
+}
+int launch_slot_scatter(const SlotOps *src, int n, int2 *ops, int32_t *flag, hipStream_t st) {
+  hipsim::launch(st, {src, ops, flag}, [=] {
+    for (int r = 0; r < n; ++r) {
+      for (int k = 0; k < 8; ++k) ops[(size_t)src[r].slot * 8 + k] = src[r].ops[k];
+      if (flag) flag[src[r].slot] = src[r].flag;
     }
   });
   return 0;

@@ -133,15 +133,27 @@ struct CopyDesc {
   uint64_t bytes;
 };
 
+// An upload's record of one slot whose ghost ops (or leaf flags) changed:
+// the slot, its leaf flags (LeafRec::flags) and its 8 ops.
+struct SlotOps {
+  int32_t slot, flag;
+  int2 ops[8];
+};
+
 // launch wrappers (wx_kernels.hip).  P and S are arrays indexed by
 // LeafRec::inst (one element for a single mesh); `grid` is the number of
 // workgroups of the persistent stage kernels.
-// Compacts the per-instance regions (records and both op tables, instance i
-// at region offset leafbase[i], inst_off[i+1] - inst_off[i] entries) into
-// the queue arrays in instance order (fill I's ops only when ops1 is set).
+// Builds the queue in instance order: entry inst_off[i] + l is leaf l of
+// instance i, whose local block ID is reg_ids[leafbase[i] + l] and whose
+// slot is slotbase[i] + that ID; its record comes from binfo and slot_flag,
+// its ops from the per-slot tables (fill I's only when ops1 is set).
 int launch_build_queue(const QueueHdr *hdr, const int32_t *inst_off, const int32_t *leafbase,
-                       const LeafRec *reg_rec, const int2 *reg_ops2, const int2 *reg_ops1, LeafRec *rec,
+                       const int32_t *slotbase, const int32_t *reg_ids, const BlockRec *binfo,
+                       const int32_t *slot_flag, const int2 *slot_ops2, const int2 *slot_ops1, LeafRec *rec,
                        int2 *ops2, int2 *ops1, int grid, hipStream_t stream);
+// Writes n SlotOps records (src may be mapped pinned memory) into the
+// per-slot op table ops (8 per slot) and, when flag is set, the flags.
+int launch_slot_scatter(const SlotOps *src, int n, int2 *ops, int32_t *flag, hipStream_t stream);
 // bs = threads per workgroup: 256, 384 or 512 (4, 6 or 8 waves).  persist
 // = 0: workgroup g takes queue entry g (grid >= queue length); 1: workgroup
 // g takes entries g, g + grid, ... (any grid).

@@ -4,10 +4,13 @@
 // Every instance keeps its own host tree, ghost list, parameters, time and
 // timestep (the reference's whole time loop, src/main.f90:69-106, run once
 // per instance).  On the device all instances share one slot space: the
-// block ID `id` of instance i lives in tile slot slotbase[i] + id, and its
-// leaf records and ghost ops in the region starting at leafbase[i].  After a
-// mesh change only the regions of instances whose mesh changed are
-// uploaded, and k_build_queue gathers all regions into the compacted queue.
+// block ID `id` of instance i lives in tile slot slotbase[i] + id, which
+// also indexes its block record, leaf flags and ghost ops; instance i's leaf
+// IDs are in the region starting at leafbase[i].  After a mesh change the
+// leaf IDs of the instances whose mesh changed go up, with the ops and flags
+// of the slots where they differ from what the device holds (a refinement
+// changes those of a few blocks, while every later leaf moves in the list),
+// and k_build_queue gathers the queue from the regions and the slots.
 // Each stage is ONE launch over that queue (one workgroup per entry, or a
 // persistent grid looping over it).  The step's launches (dt reduction and
 // the three stages, the last of which also writes the flags for the host)
@@ -47,10 +50,13 @@ struct EnsGroup : Runtime {
   DevBuf<wx_params> dP;
   DevBuf<DevScalars> dS;
   DevBuf<QueueHdr> dQ;
-  DevBuf<int32_t> dInstOff, dLeafBase;
-  DevBuf<LeafRec> dRegRec, dRec;          // per-instance regions, compacted queue
+  DevBuf<int32_t> dInstOff, dLeafBase, dSlotBase;
+  DevBuf<int32_t> dRegIds;                // per-instance regions of leaf IDs
+  DevBuf<LeafRec> dRec;                   // the compacted queue
   DevBuf<BlockRec> dBinfo;
-  DevBuf<int2> dRegOpsII, dRegOpsI, dOpsII, dOpsI;
+  DevBuf<int32_t> dSlotFlag;              // per slot: the leaf flags
+  DevBuf<int2> dSlotOpsII, dSlotOpsI;     // per slot: the ops of fills II and I
+  DevBuf<int2> dOpsII, dOpsI;             // the queue's ops
   // no instance in the OUTFYB compat mode: fill I's ops equal fill II's for
   // every instance, so they are neither uploaded nor gathered, and the fill
   // reads fill II's (fixed by the parameters for the ensemble's life)
@@ -59,6 +65,17 @@ struct EnsGroup : Runtime {
   // ID's level and position never change once created, so after a mesh
   // change only the IDs above the old maximum go up)
   std::vector<size_t> binfo_dev;
+  // per instance: the per-slot ops and flags the device holds, by local
+  // block ID (local source IDs; {-1, -1} / -1 where nothing went up yet;
+  // invalid while an upload is being prepared or after it failed), and the
+  // IDs of the current upload's changed slots
+  struct SlotShadow {
+    std::vector<int2> ops2, ops1;
+    std::vector<int32_t> flag;
+    bool valid = false;
+  };
+  std::vector<SlotShadow> shadow;
+  std::vector<std::vector<int32_t>> changed;
   const int2 *opsI() const { return opsI_same_all ? dOpsII.ptr : dOpsI.ptr; }
   DevBuf<double> dStat;
   DevBuf<int8_t> dFlag;
@@ -135,72 +152,128 @@ int graph_span(const EnsGroup *e) {
   return (int)std::min(e->nrec, std::max<size_t>(want, 256));
 }
 
-// Recompiles the tables of instances whose mesh changed and uploads their
-// regions, the leaf-count prefix sums, and rebuilds the queue index.
+// The slots of instance i whose ops (either fill's when `two`) or leaf
+// flags differ from the device's copy: their IDs go to changed[i] and the
+// shadow takes the new values.  The shadow stays invalid until the upload
+// has been launched.
+void slot_diff(EnsGroup *e, int i, bool two) {
+  const Mesh &m = e->m[i];
+  EnsGroup::SlotShadow &sh = e->shadow[i];
+  if (!sh.valid) {   // nothing known of the device's copy: every leaf goes up
+    sh.ops2.clear();
+    sh.ops1.clear();
+    sh.flag.clear();
+  }
+  sh.valid = false;
+  const size_t need = (size_t)m.nbmax + 1;
+  if (sh.flag.size() < need) {
+    sh.ops2.resize(need * 8, int2{-1, -1});
+    if (two) sh.ops1.resize(need * 8, int2{-1, -1});
+    sh.flag.resize(need, -1);
+  }
+  std::vector<int32_t> &ch = e->changed[i];
+  ch.clear();
+  for (size_t l = 0; l < m.leaves.size(); ++l) {
+    const int id = m.leaves[l].id;
+    const int2 *o2 = &m.opsII[l * 8];
+    int2 *s2 = &sh.ops2[(size_t)id * 8];
+    bool diff = sh.flag[id] != m.leaves[l].flags || memcmp(o2, s2, 8 * sizeof(int2)) != 0;
+    if (two) diff = diff || memcmp(&m.opsI[l * 8], &sh.ops1[(size_t)id * 8], 8 * sizeof(int2)) != 0;
+    if (!diff) continue;
+    memcpy(s2, o2, 8 * sizeof(int2));
+    if (two) memcpy(&sh.ops1[(size_t)id * 8], &m.opsI[l * 8], 8 * sizeof(int2));
+    sh.flag[id] = m.leaves[l].flags;
+    ch.push_back(id);
+  }
+}
+
+// Recompiles the tables of instances whose mesh changed, uploads their leaf
+// IDs, the changed slots' ops and flags, the new block records and the
+// leaf-count prefix sums, and rebuilds the queue.
 int ensure_queue(EnsGroup *e) {
   std::vector<int> stale;
   for (int i = 0; i < e->n; ++i)
     if (!e->m[i].tables_valid) stale.push_back(i);
   if (stale.empty()) return WX_OK;
   HostTimer ht(e, HID_TABLES);
+  const bool two = !e->opsI_same_all;
   e->pool->run((int)stale.size(), [&](int k) {
     const int i = stale[k];
     e->rcs[i] = refresh_tables(&e->m[i]);
-    if (e->rcs[i]) e->errs[i] = last_error();
+    if (e->rcs[i]) {
+      e->errs[i] = last_error();
+      e->shadow[i].valid = false;
+      return;
+    }
+    slot_diff(e, i, two);
   });
   int rc = collect_errors(e, stale);
   if (rc) return rc;
   e->inst_off[0] = 0;
   for (int i = 0; i < e->n; ++i) e->inst_off[i + 1] = e->inst_off[i] + (int32_t)e->m[i].leaves.size();
   e->nleaf_total = e->inst_off[e->n];
-  const bool two = !e->opsI_same_all;
   // block records from binfo_dev[i] on (every ID above the old maximum)
   auto bi_first = [&](int i) { return std::min(e->binfo_dev[i], e->m[i].binfo.size()); };
-  size_t total = al256(sizeof(QueueHdr)) + al256((size_t)(e->n + 1) * sizeof(int32_t));
-  for (int i : stale) {
-    const size_t nl = e->m[i].leaves.size();
-    total += al256(nl * sizeof(LeafRec)) + (two ? 2 : 1) * al256(nl * 8 * sizeof(int2)) +
+  std::vector<size_t> roff(stale.size() + 1, 0);   // first changed-slot record of each instance
+  for (size_t k = 0; k < stale.size(); ++k) roff[k + 1] = roff[k] + e->changed[stale[k]].size();
+  const size_t nchanged = roff.back();
+  size_t total = al256(sizeof(QueueHdr)) + al256((size_t)(e->n + 1) * sizeof(int32_t)) +
+                 (two ? 2 : 1) * al256(nchanged * sizeof(SlotOps));
+  for (int i : stale)
+    total += al256(e->m[i].leaves.size() * sizeof(int32_t)) +
              al256((e->m[i].binfo.size() - bi_first(i)) * sizeof(BlockRec));
-  }
   if ((rc = stage_begin(e, total))) return rc;
-  struct Chunk { char *rec, *o2, *o1, *bi; };
-  std::vector<Chunk> ch(stale.size());
+  SlotOps *r2 = (SlotOps *)stage_take(e, nchanged * sizeof(SlotOps));
+  SlotOps *r1 = two ? (SlotOps *)stage_take(e, nchanged * sizeof(SlotOps)) : nullptr;
+  if (!r2 || (two && !r1)) return fail(WX_EARG, "staging %zu slot records", nchanged);
+  struct Chunk { char *ids, *bi; };
+  std::vector<Chunk> chk(stale.size());
   for (size_t k = 0; k < stale.size(); ++k) {
     const Mesh &m = e->m[stale[k]];
-    const size_t nl = m.leaves.size();
-    ch[k].rec = stage_take(e, nl * sizeof(LeafRec));
-    ch[k].o2 = stage_take(e, nl * 8 * sizeof(int2));
-    ch[k].o1 = two ? stage_take(e, nl * 8 * sizeof(int2)) : nullptr;
-    ch[k].bi = stage_take(e, (m.binfo.size() - bi_first(stale[k])) * sizeof(BlockRec));
+    chk[k].ids = stage_take(e, m.leaves.size() * sizeof(int32_t));
+    chk[k].bi = stage_take(e, (m.binfo.size() - bi_first(stale[k])) * sizeof(BlockRec));
+    if (!chk[k].ids || !chk[k].bi) return fail(WX_EARG, "staging the regions of instance %d", e->ibase + stale[k]);
   }
   e->pool->run((int)stale.size(), [&](int k) {   // local IDs -> global slots
     const int i = stale[k];
     const Mesh &m = e->m[i];
     const int sb = e->slotbase[i];
-    LeafRec *r = (LeafRec *)ch[k].rec;
-    for (size_t l = 0; l < m.leaves.size(); ++l) {
-      r[l] = m.leaves[l];
-      r[l].id += sb;
-      r[l].inst = i;
+    int32_t *ids = (int32_t *)chk[k].ids;
+    for (size_t l = 0; l < m.leaves.size(); ++l) ids[l] = m.leaves[l].id;
+    const EnsGroup::SlotShadow &sh = e->shadow[i];
+    const std::vector<int32_t> &ch = e->changed[i];
+    for (size_t c = 0; c < ch.size(); ++c) {
+      const size_t id = (size_t)ch[c];
+      SlotOps &o = r2[roff[k] + c];
+      o.slot = sb + (int32_t)id;
+      o.flag = sh.flag[id];
+      for (int t = 0; t < 8; ++t) o.ops[t] = int2{sh.ops2[id * 8 + t].x + sb, sh.ops2[id * 8 + t].y};
+      if (r1) {
+        SlotOps &o1 = r1[roff[k] + c];
+        o1.slot = o.slot;
+        o1.flag = o.flag;
+        for (int t = 0; t < 8; ++t) o1.ops[t] = int2{sh.ops1[id * 8 + t].x + sb, sh.ops1[id * 8 + t].y};
+      }
     }
-    int2 *o2 = (int2 *)ch[k].o2, *o1 = (int2 *)ch[k].o1;
-    for (size_t t = 0; t < m.opsII.size(); ++t) o2[t] = int2{m.opsII[t].x + sb, m.opsII[t].y};
-    if (o1)
-      for (size_t t = 0; t < m.opsI.size(); ++t) o1[t] = int2{m.opsI[t].x + sb, m.opsI[t].y};
     const size_t b0 = bi_first(i);
-    memcpy(ch[k].bi, m.binfo.data() + b0, (m.binfo.size() - b0) * sizeof(BlockRec));
+    memcpy(chk[k].bi, m.binfo.data() + b0, (m.binfo.size() - b0) * sizeof(BlockRec));
   });
+  if (nchanged) {   // read in place from the staging (stage_end guards it)
+    rc = launch_slot_scatter((const SlotOps *)stage_dev(e, (const char *)r2), (int)nchanged, e->dSlotOpsII.ptr,
+                             e->dSlotFlag.ptr, e->stream);
+    if (!rc && two)
+      rc = launch_slot_scatter((const SlotOps *)stage_dev(e, (const char *)r1), (int)nchanged, e->dSlotOpsI.ptr,
+                               nullptr, e->stream);
+    if (rc) return fail(rc, "slot scatter launch failed");
+  }
   for (size_t k = 0; k < stale.size(); ++k) {
     const int i = stale[k];
     const Mesh &m = e->m[i];
-    const size_t nl = m.leaves.size();
-    if ((rc = stage_push(e, e->dRegRec.ptr + e->leafbase[i], ch[k].rec, nl * sizeof(LeafRec)))) return rc;
-    if ((rc = stage_push(e, e->dRegOpsII.ptr + (size_t)e->leafbase[i] * 8, ch[k].o2, nl * 8 * sizeof(int2)))) return rc;
-    if (two && (rc = stage_push(e, e->dRegOpsI.ptr + (size_t)e->leafbase[i] * 8, ch[k].o1, nl * 8 * sizeof(int2))))
+    if ((rc = stage_push(e, e->dRegIds.ptr + e->leafbase[i], chk[k].ids, m.leaves.size() * sizeof(int32_t))))
       return rc;
     const size_t b0 = bi_first(i);
     if (m.binfo.size() > b0 &&
-        (rc = stage_push(e, e->dBinfo.ptr + e->slotbase[i] + b0, ch[k].bi, (m.binfo.size() - b0) * sizeof(BlockRec))))
+        (rc = stage_push(e, e->dBinfo.ptr + e->slotbase[i] + b0, chk[k].bi, (m.binfo.size() - b0) * sizeof(BlockRec))))
       return rc;
   }
   const QueueHdr qh{e->nleaf_total, e->n, {0, 0}};
@@ -208,9 +281,13 @@ int ensure_queue(EnsGroup *e) {
   if ((rc = stage_copy(e, e->dInstOff.ptr, e->inst_off.data(), (size_t)(e->n + 1) * sizeof(int32_t)))) return rc;
   for (int i : stale) e->binfo_dev[i] = 0;   // unknown until the copies are launched
   if ((rc = stage_end(e))) return rc;
-  for (int i : stale) e->binfo_dev[i] = e->m[i].binfo.size();
-  rc = launch_build_queue(e->dQ.ptr, e->dInstOff.ptr, e->dLeafBase.ptr, e->dRegRec.ptr, e->dRegOpsII.ptr,
-                          two ? e->dRegOpsI.ptr : nullptr, e->dRec.ptr, e->dOpsII.ptr, two ? e->dOpsI.ptr : nullptr,
+  for (int i : stale) {
+    e->binfo_dev[i] = e->m[i].binfo.size();
+    e->shadow[i].valid = true;
+  }
+  rc = launch_build_queue(e->dQ.ptr, e->dInstOff.ptr, e->dLeafBase.ptr, e->dSlotBase.ptr, e->dRegIds.ptr,
+                          e->dBinfo.ptr, e->dSlotFlag.ptr, e->dSlotOpsII.ptr, two ? e->dSlotOpsI.ptr : nullptr,
+                          e->dRec.ptr, e->dOpsII.ptr, two ? e->dOpsI.ptr : nullptr,
                           std::max(1, std::min((8 * e->nleaf_total + 255) / 256, 4096)), e->stream);
   if (rc) return fail(rc, "queue build launch failed");
   return WX_OK;
@@ -387,6 +464,8 @@ int grp_create(const wx_params *p, int32_t n, int nthreads, EnsGroup **out) {
   e->errs.resize(n);
   e->iops.resize(n);
   e->binfo_dev.assign(n, 0);
+  e->shadow.resize(n);
+  e->changed.resize(n);
   e->opsI_same_all = true;
   for (int i = 0; i < n; ++i) e->opsI_same_all = e->opsI_same_all && !lb_outfyb(p[i]);
   for (int i = 0; i < n; ++i) {
@@ -409,11 +488,13 @@ int grp_create(const wx_params *p, int32_t n, int nthreads, EnsGroup **out) {
   if (!rc) rc = e->dQ.ensure(1);
   if (!rc) rc = e->dInstOff.ensure((size_t)n + 1);
   if (!rc) rc = e->dLeafBase.ensure(n);
-  if (!rc) rc = e->dRegRec.ensure(e->nrec);
+  if (!rc) rc = e->dSlotBase.ensure(n);
+  if (!rc) rc = e->dRegIds.ensure(e->nrec);
   if (!rc) rc = e->dRec.ensure(e->nrec);
   if (!rc) rc = e->dBinfo.ensure(e->nslots);
-  if (!rc) rc = e->dRegOpsII.ensure(e->nrec * 8);
-  if (!rc && !e->opsI_same_all) rc = e->dRegOpsI.ensure(e->nrec * 8);
+  if (!rc) rc = e->dSlotFlag.ensure(e->nslots);
+  if (!rc) rc = e->dSlotOpsII.ensure(e->nslots * 8);
+  if (!rc && !e->opsI_same_all) rc = e->dSlotOpsI.ensure(e->nslots * 8);
   if (!rc) rc = e->dOpsII.ensure(e->nrec * 8);
   if (!rc && !e->opsI_same_all) rc = e->dOpsI.ensure(e->nrec * 8);
   if (!rc) rc = e->dStat.ensure(e->nslots * 2);
@@ -439,9 +520,11 @@ int grp_create(const wx_params *p, int32_t n, int nthreads, EnsGroup **out) {
   (void)hipMemsetAsync(e->dQ.ptr, 0, sizeof(QueueHdr), e->stream);
   (void)hipMemsetAsync(e->dInstOff.ptr, 0, ((size_t)n + 1) * sizeof(int32_t), e->stream);
   (void)hipMemsetAsync(e->dBinfo.ptr, 0, e->nslots * sizeof(BlockRec), e->stream);
+  (void)hipMemsetAsync(e->dSlotFlag.ptr, 0, e->nslots * sizeof(int32_t), e->stream);
   if (e->dPc.ptr) (void)hipMemsetAsync(e->dPc.ptr, 0, e->dPc.n * sizeof(double), e->stream);
   (void)hipMemcpyAsync(e->dP.ptr, p, (size_t)n * sizeof(wx_params), hipMemcpyHostToDevice, e->stream);
   (void)hipMemcpyAsync(e->dLeafBase.ptr, e->leafbase.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, e->stream);
+  (void)hipMemcpyAsync(e->dSlotBase.ptr, e->slotbase.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, e->stream);
   if (hipStreamSynchronize(e->stream) != hipSuccess) { grp_destroy(e); return fail(WX_EHIP, "init sync"); }
   *out = e;
   return WX_OK;
@@ -453,8 +536,8 @@ int grp_destroy(EnsGroup *e) {
   drop_graphs(e);
   for (auto &b : e->U) b.release();
   e->dP.release(); e->dS.release(); e->dQ.release(); e->dInstOff.release(); e->dLeafBase.release();
-  e->dRegRec.release(); e->dRec.release(); e->dBinfo.release(); e->dRegOpsII.release(); e->dRegOpsI.release();
-  e->dOpsII.release(); e->dOpsI.release();
+  e->dSlotBase.release(); e->dRegIds.release(); e->dRec.release(); e->dBinfo.release(); e->dSlotFlag.release();
+  e->dSlotOpsII.release(); e->dSlotOpsI.release(); e->dOpsII.release(); e->dOpsI.release();
   e->dStat.release(); e->dFlag.release(); e->dSur.release(); e->dPc.release();
   e->Runtime::destroy();
   if (e->hS) (void)hipHostFree(e->hS);
@@ -472,6 +555,7 @@ int grp_basegrid(EnsGroup *e, int32_t inst, const double *uroots, int32_t nroots
   if ((rc = basegrid_tree(&m))) return rc;
   m.pc_stale = true;
   e->binfo_dev[inst] = 0;
+  e->shadow[inst].valid = false;
   const size_t bytes = (size_t)(nroots + 1) * e->tile() * sizeof(double);
   HIPCHK(hipMemcpyAsync(e->cur_u() + (size_t)e->slotbase[inst] * e->tile(), uroots, bytes, hipMemcpyHostToDevice,
                         e->stream));

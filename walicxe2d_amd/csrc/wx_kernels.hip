@@ -811,9 +811,12 @@ __global__ __launch_bounds__(1024) void k_dtreduce(const wx_params *__restrict__
 __global__ __launch_bounds__(256) void k_build_queue(const QueueHdr *__restrict__ hdr,
                                                      const int32_t *__restrict__ inst_off,
                                                      const int32_t *__restrict__ leafbase,
-                                                     const LeafRec *__restrict__ reg_rec,
-                                                     const int2 *__restrict__ reg_ops2,
-                                                     const int2 *__restrict__ reg_ops1,
+                                                     const int32_t *__restrict__ slotbase,
+                                                     const int32_t *__restrict__ reg_ids,
+                                                     const BlockRec *__restrict__ binfo,
+                                                     const int32_t *__restrict__ slot_flag,
+                                                     const int2 *__restrict__ slot_ops2,
+                                                     const int2 *__restrict__ slot_ops1,
                                                      LeafRec *__restrict__ rec, int2 *__restrict__ ops2,
                                                      int2 *__restrict__ ops1) {
   const int n = hdr->nleaf, ni = hdr->ninst;
@@ -825,10 +828,26 @@ __global__ __launch_bounds__(256) void k_build_queue(const QueueHdr *__restrict_
       if (inst_off[mid] <= l) lo = mid;
       else hi = mid - 1;
     }
-    const size_t src = (size_t)leafbase[lo] + (size_t)(l - inst_off[lo]);
-    if (k == 0) rec[l] = reg_rec[src];
-    ops2[(size_t)l * 8 + k] = reg_ops2[src * 8 + k];
-    if (ops1) ops1[(size_t)l * 8 + k] = reg_ops1[src * 8 + k];
+    const int lp = l - inst_off[lo];
+    const size_t slot = (size_t)slotbase[lo] + (size_t)reg_ids[(size_t)leafbase[lo] + lp];
+    if (k == 0) {
+      const BlockRec b = binfo[slot];
+      rec[l] = LeafRec{(int32_t)slot, b.level, b.icx, b.icy, slot_flag[slot], lp, lo, 0};
+    }
+    ops2[(size_t)l * 8 + k] = slot_ops2[slot * 8 + k];
+    if (ops1) ops1[(size_t)l * 8 + k] = slot_ops1[slot * 8 + k];
+  }
+}
+
+// The changed per-slot ops (and, with flag != nullptr, leaf flags) of an
+// upload, read in place from the pinned staging.
+__global__ __launch_bounds__(256) void k_slot_scatter(const SlotOps *__restrict__ src, int n,
+                                                      int2 *__restrict__ ops, int32_t *__restrict__ flag) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < 8 * n; t += gridDim.x * blockDim.x) {
+    const int r = t >> 3, k = t & 7;
+    const size_t slot = (size_t)src[r].slot;
+    ops[slot * 8 + k] = src[r].ops[k];
+    if (flag && k == 0) flag[slot] = src[r].flag;
   }
 }
 
@@ -1016,11 +1035,19 @@ __global__ void k_calcprim(const wx_params *__restrict__ p, int cap, const doubl
 static inline int hip_status() { return hipGetLastError() == hipSuccess ? WX_OK : WX_EHIP; }
 
 int launch_build_queue(const QueueHdr *hdr, const int32_t *inst_off, const int32_t *leafbase,
-                       const LeafRec *reg_rec, const int2 *reg_ops2, const int2 *reg_ops1, LeafRec *rec,
+                       const int32_t *slotbase, const int32_t *reg_ids, const BlockRec *binfo,
+                       const int32_t *slot_flag, const int2 *slot_ops2, const int2 *slot_ops1, LeafRec *rec,
                        int2 *ops2, int2 *ops1, int grid, hipStream_t stream) {
   if (grid <= 0) return WX_OK;
-  hipLaunchKernelGGL(k_build_queue, dim3(grid), dim3(256), 0, stream, hdr, inst_off, leafbase, reg_rec, reg_ops2,
-                     reg_ops1, rec, ops2, ops1);
+  hipLaunchKernelGGL(k_build_queue, dim3(grid), dim3(256), 0, stream, hdr, inst_off, leafbase, slotbase, reg_ids,
+                     binfo, slot_flag, slot_ops2, slot_ops1, rec, ops2, ops1);
+  return hip_status();
+}
+
+int launch_slot_scatter(const SlotOps *src, int n, int2 *ops, int32_t *flag, hipStream_t stream) {
+  if (n <= 0) return WX_OK;
+  const int grid = min(max(1, (8 * n + 255) / 256), 1024);
+  hipLaunchKernelGGL(k_slot_scatter, dim3(grid), dim3(256), 0, stream, src, n, ops, flag);
   return hip_status();
 }
 
