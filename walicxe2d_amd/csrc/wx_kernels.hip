@@ -668,6 +668,20 @@ __global__ __launch_bounds__(BS, (SUR ? 2 : (PERSIST ? 1024 / BS : kFillWaves)))
 #pragma unroll
     for (int k = 0; k < NEQ; ++k) tile[k * NC + IDX(gi, gj)] = g[k];
   }
+  // ghost layer 2 keeps stage B's U^{n+1/2} ring (already in Uout); the
+  // source is imposed on its cells where listed (before the cooling, which
+  // writes other cells, so that the source speed is not held across it)
+  if (imp)
+    for (int g = tid; g < 64; g += blockDim.x) {
+      int gi, gj;
+      ghost_coords(g >> 4, 2, (g & 15) + 1, gi, gj);
+      if (in_jet(p, L.level, L.icx, L.icy, gi, gj)) {
+        double v[NEQ];
+        jet_state<NEQ>(p, vjet, v);
+#pragma unroll
+        for (int k = 0; k < NEQ; ++k) ob[k * NC + IDX(gi, gj)] = v[k];
+      }
+    }
   // cooling over cells 0..n+1 (src/coolingh.f90:19-20, src/coolingdmc.f90:23-24)
   // except the four corner cells, which nothing reads: the interior (cell
   // c < 256) and the non-corner ghosts of layer 1 (c = 256 + the gather's
@@ -686,39 +700,46 @@ __global__ __launch_bounds__(BS, (SUR ? 2 : (PERSIST ? 1024 / BS : kFillWaves)))
     }
   }
   const double dts = S->dt * p->tsc;
-  for (int c = tid; c < 320; c += blockDim.x) {
-    int i, j;
-    cool_cell(c, i, j);
-    double uu[NEQ];
+  // cell c of this thread: cooled state to Uout, its primitives to the tile
+  // (under DMC with the cooled pressure, carried into the next step,
+  // src/coolingdmc.f90:27-38)
+  auto cool_pass = [&](auto cool) {
+    for (int c = tid; c < 320; c += blockDim.x) {
+      int i, j;
+      cool_cell(c, i, j);
+      double uu[NEQ];
 #pragma unroll
-    for (int k = 0; k < NEQ; ++k) uu[k] = tile[k * NC + IDX(i, j)];
-    double pcool = 0.0;
-    if (!done && mode == 1) atomic_h<NEQ>(p, dts, uu);
-    else if (!done && mode == 2) cool_dmc<NEQ>(p, dts, uu, pcool);
+      for (int k = 0; k < NEQ; ++k) uu[k] = tile[k * NC + IDX(i, j)];
+      const double pcool = cool(uu);
 #pragma unroll
-    for (int k = 0; k < NEQ; ++k) ob[k * NC + IDX(i, j)] = uu[k];
-    double q[NEQ], T;
-    uprim<NEQ>(p, q, uu, T);
-    if (mode == 2) {   // DMC: the cooled pressure, carried into the next step (src/coolingdmc.f90:27-38)
-      q[3] = pcool;
-      if (Pc) Pc[(size_t)L.id * NC + IDX(i, j)] = pcool;
-    }
-#pragma unroll
-    for (int k = 0; k < NEQ; ++k) tile[k * NC + IDX(i, j)] = q[k];
-  }
-  // ghost layer 2 keeps stage B's U^{n+1/2} ring (already in Uout); the
-  // source is imposed on its cells where listed
-  if (imp)
-    for (int g = tid; g < 64; g += blockDim.x) {
-      int gi, gj;
-      ghost_coords(g >> 4, 2, (g & 15) + 1, gi, gj);
-      if (in_jet(p, L.level, L.icx, L.icy, gi, gj)) {
-        double v[NEQ];
-        jet_state<NEQ>(p, vjet, v);
-#pragma unroll
-        for (int k = 0; k < NEQ; ++k) ob[k * NC + IDX(gi, gj)] = v[k];
+      for (int k = 0; k < NEQ; ++k) ob[k * NC + IDX(i, j)] = uu[k];
+      double q[NEQ], T;
+      uprim<NEQ>(p, q, uu, T);
+      if (mode == 2) {
+        q[3] = pcool;
+        if (Pc) Pc[(size_t)L.id * NC + IDX(i, j)] = pcool;
       }
+#pragma unroll
+      for (int k = 0; k < NEQ; ++k) tile[k * NC + IDX(i, j)] = q[k];
     }
+  };
+  // one loop per cooling mode, so that what the compiler hoists out of one
+  // mode's loop (constants of DMC's log10, kept in scratch memory at this
+  // kernel's register budget) is set up only when that mode runs
+  if (!done && mode == 1) {
+    cool_pass([&](double *uu) {
+      atomic_h<NEQ>(p, dts, uu);
+      return 0.0;
+    });
+  } else if (!done && mode == 2) {
+    cool_pass([&](double *uu) {
+      double pcool;
+      cool_dmc<NEQ>(p, dts, uu, pcool);
+      return pcool;
+    });
+  } else {
+    cool_pass([](double *) { return 0.0; });
+  }
   __syncthreads();
   block_stats<NEQ>(p, tile, L, stat, flag, fo);
   __syncthreads();   // the next queue entry reuses the LDS tile

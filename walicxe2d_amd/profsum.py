@@ -13,9 +13,9 @@ that `rocprofv3 -d DIR -o NAME` writes).
         `--pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs, each counter scaled by
         what it reads for a plain device copy of --copy-mib MiB per dispatch.
 
-A hot-path kernel launches one 256-thread workgroup per leaf block, so its
-grid size (threads) equals the interior cells it advances: the zone-updates
-of that launch.
+A hot-path kernel launches one workgroup per leaf block (of 256 threads, or
+512 for stage A), so its workgroup count times the 256 interior cells of a
+block is the zone-updates of that launch.
 """
 from __future__ import annotations
 
@@ -109,11 +109,12 @@ def valu(dbs, clock_ghz: float = 2.4, simds: int = 1024) -> str:
 def _per_kernel(db: str):
     cur = sqlite3.connect(db).cursor()
     acc = collections.defaultdict(lambda: [0.0, 0, 0])
-    for name, val, gs in cur.execute("select kernel_name, value, grid_size from counters_collection"):
+    for name, val, gs, ws in cur.execute("select kernel_name, value, grid_size, workgroup_size "
+                                         "from counters_collection"):
         a = acc[name]
         a[0] += val
         a[1] += 1
-        a[2] += gs
+        a[2] += gs // max(ws, 1) * 256   # zone-updates: one leaf block per workgroup
     return acc
 
 
