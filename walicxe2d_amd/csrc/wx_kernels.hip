@@ -354,27 +354,44 @@ __global__ __launch_bounds__(BS) void k_stageB(const wx_params *__restrict__ P, 
   int err = flux_sweep<NEQ, 2>(p, prim, F, G);
   __syncthreads();
   const int i = tid % 16 + 1, j = tid / 16 + 1;
+  double Sc[NEQ];
+  if (tid < 256 && p->cylindrical) cyl_source<NEQ>(p, prim, i, j, L.icy, L.level, Sc);
+  // the viscosity's stencil (src/viscosity.f90:6-25, on the finest two
+  // levels, src/tstep.f90:82) reads Uⁿ of the interior and of ghost layer 1
+  // from the primitive tile, which is free after the sweep
+  const bool visc = L.level >= p->nlevs - 1;
+  if (visc) {
+    __syncthreads();   // the sweep's and the sources' reads of the primitives are done
+    if (tid < 256)
+#pragma unroll
+      for (int k = 0; k < NEQ; ++k) prim[k * NC + IDX(i, j)] = u0[k];
+    for (int g = tid; g < 64; g += blockDim.x) {
+      int ri, rj;
+      ghost_coords(g >> 4, 1, (g & 15) + 1, ri, rj);
+#pragma unroll
+      for (int k = 0; k < NEQ; ++k) prim[k * NC + IDX(ri, rj)] = ub[k * NC + IDX(ri, rj)];
+    }
+    __syncthreads();
+  }
   double un[NEQ];
   if (tid < 256) {   // one thread per interior cell
     const double dt = S->dt;
     const double dtdx = dt / p->dx[L.level - 1];
     const double dtdy = dt / p->dy[L.level - 1];
-    double Sc[NEQ];
-    if (p->cylindrical) cyl_source<NEQ>(p, prim, i, j, L.icy, L.level, Sc);
-    const bool visc = L.level >= p->nlevs - 1;   // src/tstep.f90:82
     const double eta = p->eta;
 #pragma unroll
     for (int k = 0; k < NEQ; ++k) {
-      const double *uk = ub + k * NC;
+      const double *uk = prim + k * NC;
       double v = u0[k] - dtdx * (F[k * 272 + (j - 1) * 17 + i] - F[k * 272 + (j - 1) * 17 + i - 1])
                  - dtdy * (G[k * 272 + j * 16 + (i - 1)] - G[k * 272 + (j - 1) * 16 + (i - 1)]);
       if (p->cylindrical) v = v + dt * Sc[k];
-      if (visc)   // src/viscosity.f90:6-25 (no dt factor)
+      if (visc)   // no dt factor
         v = v + eta * (uk[IDX(i + 1, j)] + uk[IDX(i - 1, j)] + uk[IDX(i, j + 1)] + uk[IDX(i, j - 1)] - 4. * u0[k]);
       un[k] = v;
     }
   }
-  __syncthreads();   // every read of Uⁿ in this tile is done
+  // each thread writes the one interior cell of Uⁿ it read (the stencil's
+  // reads of other cells were from the tile in LDS)
   if (tid < 256)
 #pragma unroll
     for (int k = 0; k < NEQ; ++k) ub[k * NC + IDX(i, j)] = un[k];
