@@ -187,9 +187,12 @@ void slot_diff(EnsGroup *e, int i, bool two) {
   }
 }
 
+int upload_queue(EnsGroup *e, const std::vector<int> &stale, bool two);
+
 // Recompiles the tables of instances whose mesh changed, uploads their leaf
 // IDs, the changed slots' ops and flags, the new block records and the
-// leaf-count prefix sums, and rebuilds the queue.
+// leaf-count prefix sums, and rebuilds the queue.  On failure the changed
+// instances stay stale, with nothing assumed of the device's copy.
 int ensure_queue(EnsGroup *e) {
   std::vector<int> stale;
   for (int i = 0; i < e->n; ++i)
@@ -197,6 +200,16 @@ int ensure_queue(EnsGroup *e) {
   if (stale.empty()) return WX_OK;
   HostTimer ht(e, HID_TABLES);
   const bool two = !e->opsI_same_all;
+  const int rc = upload_queue(e, stale, two);
+  if (rc)
+    for (int i : stale) {
+      e->m[i].tables_valid = false;
+      e->shadow[i].valid = false;
+    }
+  return rc;
+}
+
+int upload_queue(EnsGroup *e, const std::vector<int> &stale, bool two) {
   e->pool->run((int)stale.size(), [&](int k) {
     const int i = stale[k];
     e->rcs[i] = refresh_tables(&e->m[i]);
