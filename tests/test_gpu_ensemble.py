@@ -85,6 +85,17 @@ def test_jet_variant_ensemble_matches_single_runs():
     _run_pair(cfgs, 8, "c5")
 
 
+def test_mixed_outfyb_ensemble_matches_single_runs():
+    """An ensemble where only some instances are in the OUTFYB compat mode
+    (outflow bottom): every instance's fill I ops go up and are gathered, the
+    others' as copies of fill II's; the per-slot uploads follow 12 mesh
+    changes of each.  Each instance equals its own single-mesh run."""
+    cfgs = [_c4(C4_PICK[0]), _c4(C4_PICK[2]), _c4(C4_PICK[3])]
+    cfgs[1].bc = ("OUTFLOW", "OUTFLOW", "REFLECTIVE", "OUTFLOW")
+    cfgs[2].bc = ("REFLECTIVE", "OUTFLOW", "REFLECTIVE", "REFLECTIVE")
+    _run_pair(cfgs, 12, "mixed outfyb")
+
+
 def _ens(cfgs, graph=False, groups=1):
     e = Ensemble(cfgs, groups=groups)
     e.basegrid()
