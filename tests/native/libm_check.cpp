@@ -26,7 +26,16 @@ static inline bool same(double a, double b) { return asu(a) == asu(b) || (a != a
 
 int main(int argc, char **argv) {
   const long n = argc > 1 ? atol(argv[1]) : 1000000;
-  long nexp = 0, npow = 0, nlog = 0, nsin = 0, nlog_cr = 0, nsin_cr = 0, ntot_log = 0;
+  long nexp = 0, npow = 0, nlog = 0, nsin = 0, nlog_cr = 0, nsin_cr = 0, ntot_log = 0, nlogpow = 0;
+  // wlog_pow(x, y) against wlog(x) and wpow(x, y), bit for bit
+  auto logpow = [&](double x, double y) {
+    double lg;
+    const double pw = wlog_pow(x, y, &lg);
+    if (!same(lg, wlog(x)) || !same(pw, wpow(x, y))) {
+      ++nlogpow;
+      printf("log_pow(%a, %a): %a %a vs %a %a\n", x, y, lg, pw, wlog(x), wpow(x, y));
+    }
+  };
   // special arguments first
   const double sp[] = {0.0, -0.0, 1.0, -1.0, 2.0, 0.5, 10.0, 1e-310, -1e-310, 0x1p-1074, 1e308, -1e308,
                        709.78, 709.79, -745.13, -745.14, -708.4, 1e-20, -1e-20, __builtin_inf(), -__builtin_inf(),
@@ -34,6 +43,7 @@ int main(int argc, char **argv) {
   const int ns = sizeof sp / sizeof sp[0];
   for (int i = 0; i < ns; ++i) {
     if (!same(exp(sp[i]), wexp(sp[i]))) ++nexp;
+    for (int j = 0; j < ns; ++j) logpow(sp[i], sp[j]);
     for (int j = 0; j < ns; ++j)
       if (!same(pow(sp[i], sp[j]), wpow(sp[i], sp[j]))) {
         ++npow;
@@ -67,6 +77,7 @@ int main(int argc, char **argv) {
       default: px = asd(next() & 0x7fffffffffffffffULL); py = (u01() - 0.5) * 8.0; break;
     }
     if (!same(pow(px, py), wpow(px, py))) ++npow;
+    logpow(px, py);
     // log on betah's arguments and on random positive doubles
     const double lx = (i % 3 == 0) ? 157890. / pow(10.0, u01() * 9.0) : asd(next() & 0x7fffffffffffffffULL);
     if (lx > 0.0 && lx < __builtin_inf()) {
@@ -84,7 +95,7 @@ int main(int argc, char **argv) {
     if (!same((double)sinq(sx), wsin(sx))) ++nsin_cr;
 #endif
   }
-  printf("n %ld exp %ld pow %ld log %ld logtotal %ld sin %ld log_cr %ld sin_cr %ld\n", n, nexp, npow, nlog, ntot_log,
-         nsin, nlog_cr, nsin_cr);
+  printf("n %ld exp %ld pow %ld log %ld logtotal %ld sin %ld log_cr %ld sin_cr %ld logpow %ld\n", n, nexp, npow, nlog,
+         ntot_log, nsin, nlog_cr, nsin_cr, nlogpow);
   return 0;
 }

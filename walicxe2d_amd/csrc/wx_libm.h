@@ -246,6 +246,26 @@ WXM_FN double wlog(double x) {
   return pow_log(ix, &tail);
 }
 
+// log(x) and pow(x, y) of one argument x, with one evaluation of pow's
+// logarithm for both (what pow computes first is what log rounds): *lg is
+// wlog(x) and the result wpow(x, y), bit for bit; the special cases of either
+// (x zero, subnormal, negative, infinite or NaN, or y outside pow's main
+// range) take the two functions themselves.
+WXM_FN double wlog_pow(double x, double y, double *lg) {
+  const uint64_t ix = asu(x);
+  const uint32_t topx = top12(x), topy = top12(y);
+  if (topx - 0x001 >= 0x7ff - 0x001 || (topy & 0x7ff) - 0x3be >= 0x43e - 0x3be) {
+    *lg = wlog(x);
+    return wpow(x, y);
+  }
+  double lo;
+  const double hi = pow_log(ix, &lo);
+  *lg = ix == asu(1.0) ? 0.0 : hi;
+  const double ehi = y * hi;
+  const double elo = fmad(y, lo, fmad(y, hi, -ehi));
+  return exp_kernel(ehi, elo, 0, true);
+}
+
 // ---------------------------------------------------------------------------
 // sin: the structure of the C library's sin (glibc >= 2.28, the IBM Accurate
 // Mathematical Library's s_sin.c after the removal of its slow paths): a

@@ -225,7 +225,9 @@ __device__ __forceinline__ double rate_alpha(double T) { return 2.55e-13 * libm:
 __device__ __forceinline__ double rate_colf(double T) { return 5.83e-11 * sqrt(T) * libm::wexp(-(157828. / T)); }
 __device__ __forceinline__ double rate_betah(double T) {
   double a = 157890. / T;
-  return 1.133e-24 / sqrt(a) * (-0.0713 + 0.5 * libm::wlog(a) + 0.640 * libm::wpow(a, -0.33333));
+  double lga;   // log(a), sharing pow's logarithm of a
+  const double pw = libm::wlog_pow(a, -0.33333, &lga);
+  return 1.133e-24 / sqrt(a) * (-0.0713 + 0.5 * lga + 0.640 * pw);
 }
 
 __device__ __forceinline__ double aloss(double x1, double den, double dh0, double te0) {
