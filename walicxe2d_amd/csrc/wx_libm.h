@@ -131,7 +131,35 @@ WXM_FN double exp_kernel(double x, double xtail, uint32_t sign_bias, bool with_t
   return fmad(scale, tmp, scale);
 }
 
-WXM_FN double wexp(double x) { return exp_kernel(x, 0.0, 0, false); }
+// The main path of exp_kernel alone, with no branch: its result for every
+// argument that takes that path (2^-54 <= |x| < 512); `special` is set for
+// the others, whose result exp_kernel gives.  (Straight-line code lets the
+// compiler interleave independent calls; the table index is in range for
+// any argument.)
+WXM_FN double exp_main(double x, double xtail, uint32_t sign_bias, bool with_tail, bool &special) {
+  const uint32_t abstop = top12(x) & 0x7ff;
+  special = special || abstop - top12(0x1p-54) >= top12(512.0) - top12(0x1p-54);
+  double kd = fmad(kInvLn2N, x, kShift);
+  const uint64_t ki = asu(kd);
+  kd -= kShift;
+  double r = fmad(kd, kNegLn2loN, fmad(kd, kNegLn2hiN, x));
+  if (with_tail) r += xtail;
+  const uint64_t idx = 2 * (ki % 128);
+  const uint64_t top = (ki + sign_bias) << 45;
+  const double tail = asd(kExpTab[idx]);
+  const uint64_t sbits = kExpTab[idx + 1] + top;
+  const double r2 = r * r;
+  const double tmp = fmad(r2 * r2, fmad(r, kC5, kC4), fmad(r2, fmad(r, kC3, kC2), tail + r));
+  const double scale = asd(sbits);
+  return fmad(scale, tmp, scale);
+}
+
+WXM_FN double wexp(double x) {
+  bool special = false;
+  double y = exp_main(x, 0.0, 0, false, special);
+  if (special) y = exp_kernel(x, 0.0, 0, false);
+  return y;
+}
 
 // pow's logarithm: log(x) = hi + tail with about 15 bits beyond double.
 // x = 2^k z, z in [OFF, 2 OFF); z/c - 1 is exact since 1/c has few bits.
@@ -182,7 +210,7 @@ WXM_FN int pow_checkint(uint64_t iy) {
 }
 WXM_FN bool pow_zeroinfnan(uint64_t i) { return 2 * i - 1 >= 2 * 0x7ff0000000000000ull - 1; }
 
-WXM_FN double wpow(double x, double y) {
+WXM_FN double wpow_full(double x, double y) {
   uint32_t sign_bias = 0;
   uint64_t ix = asu(x);
   const uint64_t iy = asu(y);
@@ -229,6 +257,25 @@ WXM_FN double wpow(double x, double y) {
   return exp_kernel(ehi, elo, sign_bias, true);
 }
 
+// pow's main path (x a positive normal number, 2^-65 <= |y| < 2^63, and exp's
+// main path), with no branch; `special` is set for the other arguments.
+WXM_FN double pow_main(double x, double y, bool &special) {
+  const uint32_t topx = top12(x), topy = top12(y);
+  special = special || topx - 0x001 >= 0x7ff - 0x001 || (topy & 0x7ff) - 0x3be >= 0x43e - 0x3be;
+  double lo;
+  const double hi = pow_log(asu(x), &lo);
+  const double ehi = y * hi;
+  const double elo = fmad(y, lo, fmad(y, hi, -ehi));
+  return exp_main(ehi, elo, 0, true, special);
+}
+
+WXM_FN double wpow(double x, double y) {
+  bool special = false;
+  double r = pow_main(x, y, special);
+  if (special) r = wpow_full(x, y);
+  return r;
+}
+
 // log: pow's extended-precision logarithm rounded to double.
 WXM_FN double wlog(double x) {
   uint64_t ix = asu(x);
@@ -254,16 +301,18 @@ WXM_FN double wlog(double x) {
 WXM_FN double wlog_pow(double x, double y, double *lg) {
   const uint64_t ix = asu(x);
   const uint32_t topx = top12(x), topy = top12(y);
-  if (topx - 0x001 >= 0x7ff - 0x001 || (topy & 0x7ff) - 0x3be >= 0x43e - 0x3be) {
-    *lg = wlog(x);
-    return wpow(x, y);
-  }
+  bool special = topx - 0x001 >= 0x7ff - 0x001 || (topy & 0x7ff) - 0x3be >= 0x43e - 0x3be;
   double lo;
   const double hi = pow_log(ix, &lo);
-  *lg = ix == asu(1.0) ? 0.0 : hi;
   const double ehi = y * hi;
   const double elo = fmad(y, lo, fmad(y, hi, -ehi));
-  return exp_kernel(ehi, elo, 0, true);
+  double r = exp_main(ehi, elo, 0, true, special);
+  *lg = ix == asu(1.0) ? 0.0 : hi;
+  if (special) {
+    *lg = wlog(x);
+    r = wpow_full(x, y);
+  }
+  return r;
 }
 
 // ---------------------------------------------------------------------------
