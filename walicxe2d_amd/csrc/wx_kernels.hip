@@ -608,9 +608,10 @@ __device__ __forceinline__ void cool_cell(int c, int &i, int &j) {
 }
 
 // Waves per SIMD the plain one-entry-per-workgroup fill is compiled for: 5
-// caps it at 96 VGPRs (a few spills), which measured faster than 4 waves at
-// 109 VGPRs (16.8 vs 17.8 ns per block on the 16-variant cooling jet).  The
-// persistent form carries the next record as well and keeps 1024 / BS.
+// caps it at 96 VGPRs (nothing spilled on its hot path), which measured
+// faster than 4 waves at 109 VGPRs (16.8 vs 17.8 ns per block on the
+// 16-variant cooling jet; 6 waves spill).  The persistent form carries the
+// next record as well and keeps 1024 / BS.
 constexpr int kFillWaves = 5;
 
 template <int NEQ, bool SUR, int BS, bool PERSIST>
