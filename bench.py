@@ -24,7 +24,7 @@ run's (the plain path's with --path plain); `plain`, `batched` and
   --batch B         jet variants in the ensemble (default 16)
   --path plain|batched   one path only (default: both)
   --no-surrogate    skip the batched+surrogate run
-  --groups G        concurrently stepped groups of the ensemble (default 4)
+  --groups G        concurrently stepped groups of the ensemble (default 2)
   --no-tune --eager default launch shapes / no hipGraphs (for counter runs)
   --c4-steps N      steps of the 64-instance explosion ensemble (C4: DMC
                     point explosions of 3..6 levels) against its instances
@@ -297,7 +297,7 @@ def _final_state(sim):
     return u, lp, leaf, ic
 
 
-def run_batched(cfgs, steps, warmup, model, nstates=2, groups=4, tune=True, graph=True):
+def run_batched(cfgs, steps, warmup, model, nstates=2, groups=2, tune=True, graph=True):
     """The batched path: every instance in one ensemble of `groups`
     concurrently stepped groups, surrogate cooling (when a model is given),
     autotuned launch shapes, steps replayed from hipGraphs.  Returns the
@@ -346,7 +346,9 @@ def main():
     ap.add_argument("--batch", type=int, default=16, help="jet variants in the batched ensemble")
     ap.add_argument("--path", choices=["both", "plain", "batched"], default="both")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the batched+surrogate run")
-    ap.add_argument("--groups", type=int, default=4, help="concurrently stepped groups of the ensemble")
+    # two groups: enough to hide each group's host work (admesh, lists)
+    # behind the other's kernels, with the larger launches of fewer groups
+    ap.add_argument("--groups", type=int, default=2, help="concurrently stepped groups of the ensemble")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes (one workgroup per block)")
     ap.add_argument("--eager", action="store_true", help="no hipGraphs: every launch from the host")
     ap.add_argument("--c4-steps", type=int, default=300, help="steps of the 64-instance C4 comparison (0: skip)")
