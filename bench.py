@@ -15,7 +15,9 @@ warm-up steps, bracketed by device synchronisation.
 Prints one JSON line: `value` is the batched path's throughput -- the
 surrogate run's when it holds the L1 bound and is faster, else the exact
 run's (the plain path's with --path plain); `plain`, `batched` and
-`surrogate` hold each run's figures.  Options:
+`surrogate` hold each run's figures; `config.stage_grids` are the tuned
+workgroup counts of stage A, stage B and the fill (0: one workgroup per leaf
+block, else a persistent grid looping over the queue).  Options:
   --gpus N          only N = 1 is supported (one GPU holds every leaf block)
   --steps K --warmup W   default K=12000, W=50: the cooling jet from t=0 to
                          about 300 yr (the configured end time), i.e. the
@@ -313,7 +315,10 @@ def run_batched(cfgs, steps, warmup, model, nstates=2, groups=2, tune=True, grap
         ens.step(warmup)
     grids, threads = [0, 0, 0], [512, 256, 256]   # the library's default shapes
     if tune:
-        grids, threads, _ = ens.tune(reps=3)
+        ens.tune(reps=3)
+        # the shapes the library keeps (group 0's): grid 0 is one workgroup
+        # per leaf block, which the tuner picks as the queue length
+        grids, threads = ens.info()[2:]
     ens.set_graph(graph)
     ens.step(0)
     ens.sur_counters(reset=True)
@@ -407,7 +412,7 @@ def main():
         bx = run_batched(cfgs, args.steps, args.warmup, None, nstates=args.batch, **bopts)
         _note(f"batched path, exact cooling: {bx['value']:.3e} zone-updates/s")
         batched = {"value": bx["value"], "ms_per_step": bx["ms_per_step"],
-                   "leaf_blocks_end_total": bx["nleaf"], "persistent_grids": bx["grids"],
+                   "leaf_blocks_end_total": bx["nleaf"], "stage_grids": bx["grids"],
                    "threads_per_workgroup": bx["threads"],
                    "roofline": roofline(bx["prof"], bx["prof_zones"], neq, "c3_batched")}
         best, best_path, best_prof = bx, "batched", batched["roofline"]
@@ -432,7 +437,7 @@ def main():
         out["dtype"] = "f64 hydro, bf16/fp32 cooling surrogate" if best_path.endswith("surrogate") else "f64"
         out["config"].update({"path": best_path, "instances": args.batch,
                               "variants": "jet seeds 777+100..777+%d" % (99 + args.batch),
-                              "leaf_blocks_end_total": best["nleaf"], "persistent_grids": best["grids"],
+                              "leaf_blocks_end_total": best["nleaf"], "stage_grids": best["grids"],
                               "threads_per_workgroup": best["threads"], "graph": not args.eager,
                               "groups": args.groups})
         out["roofline"] = best_prof
